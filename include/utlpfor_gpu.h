@@ -1,0 +1,211 @@
+/*
+ * utlpfor_gpu.h -- C-ABI of the MI355X-native UTL-PFOR block codec.
+ *
+ * Drop-in boundary for the Akron/utlpfor Go API (reference file:line given at
+ * each entry point). Plain pointers and sizes only; every compute entry point
+ * runs hand-written HIP kernels for gfx950.
+ *
+ * Error codes map 1:1 to the Go sentinels (utlpfor.go:71-115). ErrOverflow's
+ * Position is returned through an out-parameter. Codes >= 100 are failures of
+ * this library or of the HIP runtime and have no Go counterpart.
+ *
+ * Threading: package-level entry points take a utl_ctx*; passing NULL uses a
+ * per-thread default context, so they are reentrant like the Go functions.
+ * A utl_ctx (one HIP stream + staging memory) corresponds to a Go
+ * Compressor32/64 and, like it, is not safe for concurrent use
+ * (compressor.go:7-8).
+ */
+#ifndef UTLPFOR_GPU_H
+#define UTLPFOR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (utlpfor.go:71-115) ---- */
+#define UTL_OK 0
+#define UTL_ERR_INVALID_BUFFER 1        /* ErrInvalidBuffer */
+#define UTL_ERR_POSITION_OUT_OF_RANGE 2 /* ErrPositionOutOfRange */
+#define UTL_ERR_INVALID_BLOCK_LENGTH 3  /* ErrInvalidBlockLength */
+#define UTL_ERR_INVALID_FLAGS 4         /* ErrInvalidFlags */
+#define UTL_ERR_UNSUPPORTED_TYPE 5      /* ErrUnsupportedType */
+#define UTL_ERR_OVERFLOW 6              /* *ErrOverflow{Position} */
+#define UTL_ERR_ARGUMENT 100            /* bad C-ABI argument (null pointer, short capacity) */
+#define UTL_ERR_HIP 101                 /* HIP runtime failure */
+#define UTL_ERR_NO_DEVICE 102           /* no usable gfx950 device */
+
+/* Per-block status words written by the batched entry points:
+ * status = code | (overflow_position << 8). */
+#define UTL_STATUS_CODE(s) ((s) & 0xFF)
+#define UTL_STATUS_POS(s) ((s) >> 8)
+
+/* ---- flags (utlpfor.go:21-68) ---- */
+#define UTL_DELTA 1u
+#define UTL_NOFOR 2u
+#define UTL_NOPATCH 4u
+#define UTL_SPECIAL 8u
+#define UTL_APPEND 16u
+#define UTL_NOINPLACE 32u
+
+/* pack.go:5,11 and uint64.go:15 */
+#define UTL_SCRATCH_LEN 128
+#define UTL_SCRATCH_LEN_NOINPLACE 256
+#define UTL_SCRATCH_LEN64 384
+
+typedef struct utl_ctx utl_ctx;
+
+/* ---- context (compressor.go:16-107 NewUint32/NewUint64) ---- */
+/* Creates a context on `device` (HIP ordinal): one HIP stream, pinned host
+ * staging and device scratch. Returns NULL if no device is usable. */
+utl_ctx *utl_ctx_create(int device);
+void utl_ctx_destroy(utl_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work against it. */
+void *utl_ctx_stream(utl_ctx *ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+int utl_ctx_sync(utl_ctx *ctx);
+/* Human-readable device name and compute-unit count of the context. */
+int utl_ctx_device_info(utl_ctx *ctx, char *name, size_t name_cap, int *cu_count);
+
+/* ---- per-call drop-in: one reference call = one C-ABI call ---- */
+
+/* scalar.go:336 MaxBlockLength32 */
+int utl_max_block_length32(uint32_t flag);
+/* uint64.go:451 MaxBlockLength64 */
+int utl_max_block_length64(uint32_t flag);
+
+/* header.go:135 Header. out[7] = count, bitWidth, excCount, hasDelta, hasFOR,
+ * hasZigZag, hasSpecial. A 4-byte host read: no device work. */
+int utl_header(const uint8_t *src, size_t len, int32_t out[7]);
+
+/* block_length.go:8 BlockLength. A <=6-byte host read: no device work; the
+ * device form for whole streams is utl_block_lengths. */
+int utl_block_length(const uint8_t *src, size_t len, int64_t *out_len);
+
+/* pack.go:76 PackUint32. Encodes values[0:n] (1 <= n <= 128) on the GPU into
+ * dst (capacity dst_cap >= utl_max_block_length32(flag) recommended; a shorter
+ * capacity that cannot hold the block returns UTL_ERR_ARGUMENT with the
+ * needed size in *out_len). Append has no positional meaning here: the caller
+ * concatenates. Like the reference, values is overwritten with the post-FOR,
+ * post-delta, post-zigzag work values unless NoInPlace or Append is set. */
+int utl_pack_u32(utl_ctx *ctx, uint32_t flag, uint32_t *values, size_t n, uint8_t *dst, size_t dst_cap,
+                 size_t *out_len);
+
+/* unpack.go:5 UnpackUint32. Decodes one block on the GPU. dst must hold 128
+ * values; *count values are valid. *consumed as the reference returns it.
+ * On UTL_ERR_OVERFLOW, *ovf_pos = ErrOverflow.Position. */
+int utl_unpack_u32(utl_ctx *ctx, const uint8_t *src, size_t len, uint32_t *dst, int32_t *count,
+                   int64_t *consumed, int64_t *ovf_pos);
+
+/* get.go:74 GetUint32 (delta full-unpack threshold of the reference's
+ * AVX-512 dispatch level, get.go:18-28). */
+int utl_get_u32(utl_ctx *ctx, int64_t pos, const uint8_t *src, size_t len, uint32_t *out);
+
+/* uint64.go:279 PackUint64 (values are never modified). */
+int utl_pack_u64(utl_ctx *ctx, uint32_t flag, const uint64_t *values, size_t n, uint8_t *dst,
+                 size_t dst_cap, size_t *out_len);
+
+/* uint64.go:338 UnpackUint64. dst must hold 128 values. */
+int utl_unpack_u64(utl_ctx *ctx, const uint8_t *src, size_t len, uint64_t *dst, int32_t *count,
+                   int64_t *consumed, int64_t *ovf_pos);
+
+/* uint64.go:364 GetUint64 */
+int utl_get_u64(utl_ctx *ctx, int64_t pos, const uint8_t *src, size_t len, uint64_t *out);
+
+/* ---- device memory (the batched forms work on device-resident data) ---- */
+int utl_dev_alloc(void **ptr, size_t bytes);
+int utl_dev_free(void *ptr);
+int utl_memcpy_h2d(utl_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
+int utl_memcpy_d2h(utl_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
+int utl_memset_d(utl_ctx *ctx, void *dst, int value, size_t bytes);          /* fill kernel, async */
+int utl_dev_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- batched, device-resident forms (no reference counterpart; each block
+ * result equals the per-call result on the same bytes) ----
+ *
+ * A stream index describes nblocks concatenated blocks: block_off[k] is the
+ * byte offset of block k in src and block_off[k+1] its end, so block k is
+ * decoded exactly as UnpackUint32(src[block_off[k]:block_off[k+1]]), which on
+ * every block a serial BlockLength walk reaches equals the walk's result
+ * (block_length.go:8, example_test.go:171-187). out_off[k] is the value
+ * offset of block k in the decoded output. All pointers prefixed d_ are
+ * device pointers. Calls are asynchronous on the context's stream. */
+
+/* Device exclusive scan that replaces the serial BlockLength walk: from
+ * per-block meta words (byte length | value count << 24, as the encoders
+ * emit them) build block_off[nblocks+1] and out_off[nblocks+1]. */
+int utl_index_build(utl_ctx *ctx, const uint32_t *d_meta, size_t nblocks, uint64_t *d_block_off,
+                    uint64_t *d_out_off);
+
+/* Header-parse pass over a device stream whose block starts are known:
+ * BlockLength (block_length.go:8) of block k = src[start[k]:limit[k]], as
+ * meta words (length | count << 24) and per-block status. */
+int utl_block_lengths(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_start, const uint64_t *d_limit,
+                      size_t nblocks, uint32_t *d_meta, int32_t *d_status);
+
+/* Batched UnpackUint32 over an index (one launch). d_status[k] receives the
+ * block's status word; d_consumed (optional, may be NULL) its consumed
+ * bytes. Values of failed blocks are not written. */
+int utl_decode_u32(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_off, const uint64_t *d_out_off,
+                   size_t nblocks, uint32_t *d_out, int32_t *d_status, int32_t *d_consumed);
+
+/* Batched UnpackUint64 over an index (one launch). */
+int utl_decode_u64(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_off, const uint64_t *d_out_off,
+                   size_t nblocks, uint64_t *d_out, int32_t *d_status, int32_t *d_consumed);
+
+/* Persistent form of the batched decode (is_u64 selects UnpackUint64): a
+ * fixed grid of wavefronts pulls runs of blocks from a device work queue.
+ * Launch shape from utl_set_tune (chosen by the autotuner in
+ * utlpfor_amd/autotune.py); results identical to utl_decode_u32/u64. */
+int utl_decode_persistent(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
+                          const uint64_t *d_out_off, size_t nblocks, void *d_out, int32_t *d_status,
+                          int32_t *d_consumed);
+int utl_set_tune(utl_ctx *ctx, int waves_per_wg, int grid_wgs, int blocks_per_wave);
+int utl_get_tune(utl_ctx *ctx, int *waves_per_wg, int *grid_wgs, int *blocks_per_wave);
+
+/* Batched GetUint32 / GetUint64 (get.go:74, uint64.go:364): request i reads
+ * position d_req_pos[i] of block d_req_block[i]; with d_req_block NULL,
+ * d_req_pos[i] is a position in the decoded stream and its block is found by
+ * binary search over d_out_off. d_out receives 4- or 8-byte values,
+ * d_status the per-request status word. */
+int utl_get_batch(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
+                  const uint64_t *d_out_off, size_t nblocks, const uint64_t *d_req_block, const int64_t *d_req_pos,
+                  size_t nreq, void *d_out, int32_t *d_status);
+
+/* Batched PackUint32 / PackUint64 (pack.go:76, uint64.go:279) over a job
+ * table: block b encodes values[d_in_off[b]..] with
+ * d_in_meta[b] = count (1..128) | flag << 8 into slot b of d_slots
+ * (UTL_SLOT_BYTES32/64 apart); d_out_meta[b] = length | count << 24, or
+ * 0xFFFFFFFF for a count outside 1..128 (ErrInvalidBuffer). */
+#define UTL_SLOT_BYTES32 1024u
+#define UTL_SLOT_BYTES64 2048u
+int utl_encode_blocks(utl_ctx *ctx, int is_u64, const void *d_values, const uint64_t *d_in_off,
+                      const uint32_t *d_in_meta, size_t nblocks, uint8_t *d_slots, uint32_t *d_out_meta);
+/* Concatenates encoded slots into d_dst at d_block_off (from utl_index_build
+ * over d_out_meta): byte-identical to appending the blocks with Append. */
+int utl_compact(utl_ctx *ctx, int is_u64, const uint8_t *d_slots, const uint64_t *d_block_off, size_t nblocks,
+                uint8_t *d_dst);
+/* Job table for streams cut into 128-value blocks (the last one partial):
+ * stream s covers values [d_val_off[s], d_val_off[s+1]) with flag d_flags[s],
+ * its blocks numbered from d_blk_off[s]. */
+int utl_make_jobs(utl_ctx *ctx, const uint64_t *d_val_off, const uint64_t *d_blk_off, const uint8_t *d_flags,
+                  size_t nstreams, uint64_t *d_in_off, uint32_t *d_in_meta);
+
+/* HIP events on the context's stream, for timing device work in place. */
+int utl_event_create(void **ev);
+int utl_event_destroy(void *ev);
+int utl_event_record(utl_ctx *ctx, void *ev);
+int utl_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Plain device-to-device copy kernel (bandwidth calibration). */
+int utl_dev_copy(utl_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+
+/* Message of the last UTL_ERR_HIP / UTL_ERR_NO_DEVICE on this thread. */
+const char *utl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1076,13 +1076,18 @@ static void *run_job(void *arg) {
       int err;
       const uint8_t *p = j->src + j->off[b];
       size_t len = (size_t)(j->off[b + 1] - j->off[b]);
+      uint64_t slot = j->out_off[b + 1] - j->out_off[b];
+      /* a block decoding to more values than its index slot: 100 (the
+       * product's UTL_ERR_ARGUMENT), nothing written */
       if (j->kind == 2) {
         uint32_t tmp[BLOCK];
         err = orc_unpack_u32(p, len, tmp, &c, &cons, &ovf);
+        if (!err && (uint64_t)c > slot) err = 100;
         if (!err) memcpy((uint32_t *)j->out + j->out_off[b], tmp, (size_t)c * 4);
       } else {
         uint64_t tmp[BLOCK];
         err = orc_unpack_u64(p, len, tmp, &c, &cons, &ovf);
+        if (!err && (uint64_t)c > slot) err = 100;
         if (!err) memcpy((uint64_t *)j->out + j->out_off[b], tmp, (size_t)c * 8);
       }
       j->status[b] = err | (int32_t)(err == ORC_ERR_OVERFLOW ? (ovf << 8) : 0);

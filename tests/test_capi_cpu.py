@@ -1,0 +1,79 @@
+"""CPU-only checks of the C-ABI library: it loads, exports every symbol the
+header declares, and its host-side entry points (Header, BlockLength,
+MaxBlockLength: a few-byte header read each, no device work) agree with the
+oracle. No compute entry point is called without a GPU."""
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(utl_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from utlpfor_amd import _lib
+    return _lib.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 30
+    from utlpfor_amd import _lib
+    assert names <= set(_lib.SIGNATURES), names - set(_lib.SIGNATURES)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def _blocks(orc):
+    rng = np.random.default_rng(11)
+    out = []
+    for i in range(60):
+        n = int(rng.integers(1, 129))
+        v = rng.integers(0, 1 << int(rng.integers(0, 33)), n, dtype=np.uint64).astype(np.uint32)
+        out.append(orc.PackUint32(i % 16, v)[0])
+        out.append(orc.PackUint64(i % 16, rng.integers(0, 1 << 63, n, dtype=np.uint64))[0])
+    return out
+
+
+def test_host_entry_points_match_oracle(lib, orc):
+    from utlpfor_amd import codec
+    for flag in range(64):
+        assert codec.MaxBlockLength32(flag) == orc.MaxBlockLength32(flag)
+        assert codec.MaxBlockLength64(flag) == orc.MaxBlockLength64(flag)
+    rng = np.random.default_rng(3)
+    samples = _blocks(orc)
+    for _ in range(400):
+        samples.append(rng.integers(0, 256, int(rng.integers(0, 12)), dtype=np.uint8).tobytes())
+    for blk in samples:
+        for cut in {len(blk), min(len(blk), 5), min(len(blk), 3)}:
+            b = blk[:cut]
+            want = orc.Header(b)
+            got = codec.Header(b)
+            if want[-1] == 0:
+                assert got[:-1] == want[:-1] and got[-1] is None
+            else:
+                assert got[-1].code == want[-1]
+            wl, we = orc.BlockLength(b)
+            gl, ge = codec.BlockLength(b)
+            assert (ge.code if ge else 0) == we and (gl == wl or we)
+
+
+def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
+    from utlpfor_amd import _lib
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError):
+        _lib.load()

@@ -1,0 +1,156 @@
+"""Device-resident streams: batched encode, index, decode and Get.
+
+A DeviceStream is the concatenation of UTL-PFOR blocks exactly as a caller
+of the reference builds it with PackUint32(flag|Append, ...) per 128-value
+chunk (pack.go:53-76), kept in HBM together with its index: per-block meta
+words (byte length | count << 24, produced by the encoder), block byte
+offsets and decoded-value offsets (the device scan that replaces the serial
+BlockLength walk of example_test.go:171-187).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .device import Context, DeviceArray, check
+
+SLOT32, SLOT64 = 1024, 2048
+
+
+@dataclass
+class DeviceStream:
+    ctx: Context
+    data: DeviceArray       # uint8, concatenated blocks
+    meta: DeviceArray       # uint32 per block
+    block_off: DeviceArray  # uint64, nblocks + 1
+    out_off: DeviceArray    # uint64, nblocks + 1
+    nblocks: int
+    nvalues: int
+    nbytes: int
+    is64: bool
+
+    def bytes_host(self) -> np.ndarray:
+        return self.data.to_numpy(self.nbytes)
+
+    def free(self) -> None:
+        for a in (self.data, self.meta, self.block_off, self.out_off):
+            a.free()
+
+
+def plan_blocks(stream_lengths: np.ndarray) -> np.ndarray:
+    """Block numbering of streams cut into 128-value chunks: blk_off[s]."""
+    nb = (np.asarray(stream_lengths, dtype=np.uint64) + 127) // 128
+    return np.concatenate([[0], np.cumsum(nb)]).astype(np.uint64)
+
+
+def build_index(ctx: Context, meta: DeviceArray, nblocks: int):
+    lib = _lib.load()
+    boff = DeviceArray(nblocks + 1, np.uint64, ctx)
+    ooff = DeviceArray(nblocks + 1, np.uint64, ctx)
+    check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "utl_index_build")
+    return boff, ooff
+
+
+def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flags=None,
+           is64: bool | None = None) -> DeviceStream:
+    """Encodes one stream (or several, back to back) on the GPU. `values` is a
+    numpy array or a DeviceArray of uint32/uint64."""
+    lib = _lib.load()
+    dv = values if isinstance(values, DeviceArray) else DeviceArray.from_numpy(np.ascontiguousarray(values), ctx)
+    if is64 is None:
+        is64 = dv.dtype == np.uint64
+    n = dv.n
+    if stream_lengths is None:
+        stream_lengths = np.array([n], dtype=np.uint64)
+    stream_lengths = np.asarray(stream_lengths, dtype=np.uint64)
+    if stream_flags is None:
+        stream_flags = np.full(stream_lengths.size, flag, dtype=np.uint8)
+    blk_off = plan_blocks(stream_lengths)
+    nblocks = int(blk_off[-1])
+    val_off = np.concatenate([[0], np.cumsum(stream_lengths)]).astype(np.uint64)
+    d_val_off = DeviceArray.from_numpy(val_off, ctx)
+    d_blk_off = DeviceArray.from_numpy(blk_off, ctx)
+    d_flags = DeviceArray.from_numpy(np.asarray(stream_flags, dtype=np.uint8), ctx)
+    in_off = DeviceArray(nblocks, np.uint64, ctx)
+    in_meta = DeviceArray(nblocks, np.uint32, ctx)
+    check(lib.utl_make_jobs(ctx.handle, d_val_off.ptr, d_blk_off.ptr, d_flags.ptr, stream_lengths.size, in_off.ptr,
+                            in_meta.ptr), "utl_make_jobs")
+    slot = SLOT64 if is64 else SLOT32
+    slots = DeviceArray(nblocks * slot, np.uint8, ctx)
+    meta = DeviceArray(nblocks, np.uint32, ctx)
+    check(lib.utl_encode_blocks(ctx.handle, int(is64), dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr,
+                                meta.ptr), "utl_encode_blocks")
+    boff, ooff = build_index(ctx, meta, nblocks)
+    total = int(boff.to_numpy()[-1]) if nblocks else 0
+    data = DeviceArray(total + 64, np.uint8, ctx)
+    check(lib.utl_compact(ctx.handle, int(is64), slots.ptr, boff.ptr, nblocks, data.ptr), "utl_compact")
+    ctx.sync()
+    for a in (slots, in_off, in_meta, d_val_off, d_blk_off, d_flags):
+        a.free()
+    if values is not dv:
+        dv.free()
+    return DeviceStream(ctx, data, meta, boff, ooff, nblocks, n, total, bool(is64))
+
+
+def from_host_blocks(ctx: Context, stream_bytes: np.ndarray, block_len: np.ndarray, block_count: np.ndarray,
+                     is64: bool = False) -> DeviceStream:
+    """Uploads an already-encoded stream with its per-block lengths/counts."""
+    nblocks = int(block_len.size)
+    meta_h = (block_len.astype(np.uint32) & 0xFFFFFF) | (block_count.astype(np.uint32) << 24)
+    meta = DeviceArray.from_numpy(meta_h, ctx)
+    boff, ooff = build_index(ctx, meta, nblocks)
+    data = DeviceArray.from_numpy(np.concatenate([np.asarray(stream_bytes, dtype=np.uint8),
+                                                  np.zeros(64, np.uint8)]), ctx)
+    ctx.sync()
+    return DeviceStream(ctx, data, meta, boff, ooff, nblocks, int(block_count.sum()), int(stream_bytes.size),
+                        is64)
+
+
+def decode(ctx: Context, s: DeviceStream, out: DeviceArray | None = None, status: DeviceArray | None = None,
+           consumed: DeviceArray | None = None, persistent: bool = False):
+    """One launch decoding every block of the stream into `out`."""
+    lib = _lib.load()
+    if out is None:
+        out = DeviceArray(max(s.nvalues, 1), np.uint64 if s.is64 else np.uint32, ctx)
+    if status is None:
+        status = DeviceArray(max(s.nblocks, 1), np.int32, ctx)
+    cptr = consumed.ptr if consumed is not None else None
+    if persistent:
+        code = lib.utl_decode_persistent(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr,
+                                         s.nblocks, out.ptr, status.ptr, cptr)
+    elif s.is64:
+        code = lib.utl_decode_u64(ctx.handle, s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks, out.ptr,
+                                  status.ptr, cptr)
+    else:
+        code = lib.utl_decode_u32(ctx.handle, s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks, out.ptr,
+                                  status.ptr, cptr)
+    check(code, "decode")
+    return out, status
+
+
+def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: DeviceArray | None = None,
+              status: DeviceArray | None = None, blocks: DeviceArray | None = None):
+    """Batched Get of stream positions (or of (block, in-block pos) pairs)."""
+    lib = _lib.load()
+    n = positions.n
+    if out is None:
+        out = DeviceArray(max(n, 1), np.uint64 if s.is64 else np.uint32, ctx)
+    if status is None:
+        status = DeviceArray(max(n, 1), np.int32, ctx)
+    check(lib.utl_get_batch(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
+                            blocks.ptr if blocks is not None else None, positions.ptr, n, out.ptr, status.ptr),
+          "utl_get_batch")
+    return out, status
+
+
+def set_tune(ctx: Context, waves_per_wg: int, grid_wgs: int, blocks_per_wave: int) -> None:
+    check(_lib.load().utl_set_tune(ctx.handle, waves_per_wg, grid_wgs, blocks_per_wave), "utl_set_tune")
+
+
+def get_tune(ctx: Context):
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    check(_lib.load().utl_get_tune(ctx.handle, C.byref(a), C.byref(b), C.byref(c)), "utl_get_tune")
+    return a.value, b.value, c.value

@@ -1,0 +1,220 @@
+// utl_index.hip -- stream index kernels.
+//
+// The reference walks a concatenated stream serially: block k+1 starts where
+// BlockLength (block_length.go:8) of block k ends (example_test.go:171-187).
+// Here the per-block meta words (byte length | value count << 24) that the
+// encoders emit are turned into block byte offsets and output value offsets
+// by a three-pass device exclusive scan (tile reduce, tile-prefix scan, tile
+// apply), which removes the loop-carried dependency. A header-parse pass
+// (k_block_lengths) recomputes BlockLength for every block whose start is
+// known, on the device.
+#include "utl_device.h"
+#include "utl_format.h"
+#include "utl_kernels.h"
+
+namespace utl {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ void split_meta(uint32_t m, uint64_t& len, uint64_t& cnt) {
+  len = m & 0xFFFFFFu;
+  cnt = m >> 24;
+}
+
+// Exclusive scan over a 256-thread workgroup. lds holds 4 entries.
+__device__ __forceinline__ uint64_t wg_excl_scan(uint64_t x, uint64_t* lds, uint64_t& total) {
+  const int tid = threadIdx.x;
+  const uint64_t incl = wave_incl_scan64(x);
+  if ((tid & 63) == 63) lds[tid >> 6] = incl;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    const uint64_t v = lds[w];
+    if (w < (tid >> 6)) pre += v;
+    tot += v;
+  }
+  __syncthreads();
+  total = tot;
+  return incl - x + pre;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ meta, uint64_t n,
+                                                               uint64_t* __restrict__ tile_len,
+                                                               uint64_t* __restrict__ tile_cnt) {
+  __shared__ uint64_t lds[2][4];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t sl = 0, sc = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) {
+      uint64_t l, c;
+      split_meta(meta[base + i], l, c);
+      sl += l;
+      sc += c;
+    }
+  }
+  uint64_t tl, tc;
+  wg_excl_scan(sl, lds[0], tl);
+  wg_excl_scan(sc, lds[1], tc);
+  if (threadIdx.x == 0) {
+    tile_len[blockIdx.x] = tl;
+    tile_cnt[blockIdx.x] = tc;
+  }
+}
+
+// One workgroup turns tile totals into exclusive tile prefixes, in place.
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(uint64_t* __restrict__ tile_len,
+                                                              uint64_t* __restrict__ tile_cnt, uint64_t ntiles) {
+  __shared__ uint64_t lds[2][4];
+  uint64_t carry_l = 0, carry_c = 0;
+  for (uint64_t base = 0; base < ntiles; base += kScanThreads) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t l = i < ntiles ? tile_len[i] : 0, c = i < ntiles ? tile_cnt[i] : 0;
+    uint64_t tl, tc;
+    const uint64_t el = wg_excl_scan(l, lds[0], tl);
+    const uint64_t ec = wg_excl_scan(c, lds[1], tc);
+    if (i < ntiles) {
+      tile_len[i] = carry_l + el;
+      tile_cnt[i] = carry_c + ec;
+    }
+    carry_l += tl;
+    carry_c += tc;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ meta, uint64_t n,
+                                                              const uint64_t* __restrict__ tile_len,
+                                                              const uint64_t* __restrict__ tile_cnt,
+                                                              uint64_t* __restrict__ boff, uint64_t* __restrict__ ooff) {
+  __shared__ uint64_t lds[2][4];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t l[kScanItems], c[kScanItems], sl = 0, sc = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    l[i] = c[i] = 0;
+    if (base + i < n) split_meta(meta[base + i], l[i], c[i]);
+    sl += l[i];
+    sc += c[i];
+  }
+  uint64_t tl, tc;
+  uint64_t pl = wg_excl_scan(sl, lds[0], tl) + tile_len[blockIdx.x];
+  uint64_t pc = wg_excl_scan(sc, lds[1], tc) + tile_cnt[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) {
+      boff[base + i] = pl;
+      ooff[base + i] = pc;
+    }
+    pl += l[i];
+    pc += c[i];
+    if (base + i + 1 == n) {
+      boff[n] = pl;
+      ooff[n] = pc;
+    }
+  }
+}
+
+// BlockLength of every block with a known start (header-parse pass).
+__global__ void k_block_lengths(const uint8_t* __restrict__ src, const uint64_t* __restrict__ start,
+                                const uint64_t* __restrict__ limit, uint64_t n, uint32_t* __restrict__ meta,
+                                int32_t* __restrict__ status) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const uint8_t* p = src + start[b];
+  const int64_t len = (int64_t)(limit[b] - start[b]);
+  int64_t bl = 0;
+  const int st = block_length_impl(
+      len, [p](int off) { return ld_u32(p + off); }, [p](int off) { return ld_u16(p + off); }, &bl);
+  const uint32_t count = len >= kHdr ? (ld_u8(p) & 0xFFu) : 0u;
+  status[b] = st;
+  meta[b] = st == UTL_OK ? ((uint32_t)(bl & 0xFFFFFF) | (count << 24)) : 0u;
+}
+
+__global__ void k_zero_first(uint64_t* a, uint64_t* b) {
+  a[0] = 0;
+  b[0] = 0;
+}
+
+__global__ void k_fill32(uint32_t* p, uint32_t v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill8(uint8_t* p, uint8_t v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace utl
+
+using namespace utl;
+
+size_t utl_index_tmp_bytes(uint64_t nblocks) {
+  const uint64_t ntiles = (nblocks + kTile - 1) / kTile;
+  return (size_t)(2 * ntiles * sizeof(uint64_t) + 256);
+}
+
+hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, uint64_t* ooff, void* tmp,
+                            hipStream_t stream) {
+  if (n == 0) {
+    k_zero_first<<<1, 1, 0, stream>>>(boff, ooff);
+    return hipGetLastError();
+  }
+  const uint64_t ntiles = (n + kTile - 1) / kTile;
+  uint64_t* tl = (uint64_t*)tmp;
+  uint64_t* tc = tl + ntiles;
+  k_scan_reduce<<<dim3((unsigned)ntiles), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc);
+  k_scan_tiles<<<1, kScanThreads, 0, stream>>>(tl, tc, ntiles);
+  k_scan_apply<<<dim3((unsigned)ntiles), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, const uint64_t* limit, uint64_t n,
+                                    uint32_t* meta, int32_t* status, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  k_block_lengths<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream>>>(src, start, limit, n, meta, status);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  k_fill32<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream>>>(p, v, n);
+  return hipGetLastError();
+}
+
+namespace utl {
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ __launch_bounds__(256) void k_copy1(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace utl
+
+hipError_t utl_launch_copy(uint8_t* dst, const uint8_t* src, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint64_t n16 = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0 ? n / 16 : 0;
+  if (n16) {
+    const uint64_t blocks = (n16 + 255) / 256;
+    k_copy16<<<dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream>>>((uint4*)dst,
+                                                                                          (const uint4*)src, n16);
+  }
+  const uint64_t done = n16 * 16, rest = n - done;
+  if (rest) {  // tail, or the whole copy when a pointer is not 16-byte aligned
+    const uint64_t blocks = (rest + 255) / 256;
+    k_copy1<<<dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream>>>(dst + done, src + done,
+                                                                                         rest);
+  }
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  k_fill8<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream>>>(p, v, n);
+  return hipGetLastError();
+}

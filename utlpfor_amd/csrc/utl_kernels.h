@@ -1,0 +1,34 @@
+// utl_kernels.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Launch shape of the persistent decode kernel, chosen by the autotuner.
+struct utl_tune {
+  int waves_per_wg;       // wavefronts per workgroup (1, 2, 4, 8)
+  int grid_wgs;           // workgroups in the persistent grid
+  uint32_t blocks_per_wave;  // blocks a wave takes from the work queue per grab
+};
+
+hipError_t utl_launch_decode(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
+                             void* out, int32_t* status, int32_t* consumed, hipStream_t stream);
+
+hipError_t utl_launch_decode_persistent(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff,
+                                        uint64_t nblocks, void* out, int32_t* status, int32_t* consumed,
+                                        unsigned long long* counter, const utl_tune& tune, hipStream_t stream);
+
+// Exclusive scans of (length, count) meta words into block/value offsets.
+// tmp must hold utl_index_tmp_bytes(nblocks) bytes.
+size_t utl_index_tmp_bytes(uint64_t nblocks);
+hipError_t utl_launch_index(const uint32_t* meta, uint64_t nblocks, uint64_t* boff, uint64_t* ooff, void* tmp,
+                            hipStream_t stream);
+
+hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, const uint64_t* limit, uint64_t nblocks,
+                                    uint32_t* meta, int32_t* status, hipStream_t stream);
+
+hipError_t utl_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t stream);
+hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t stream);
+
+// Plain streaming copy (16-byte vectors when both pointers allow it).
+hipError_t utl_launch_copy(uint8_t* dst, const uint8_t* src, uint64_t n, hipStream_t stream);
