@@ -23,10 +23,11 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // n (1..4) little-endian bytes at p.
+// (Pointer arithmetic on p itself, not an integer round trip, keeps the
+// compiler's global address-space inference, so these stay global_load.)
 __device__ __forceinline__ uint32_t ld_bytes(const uint8_t* p, int n) {
-  uintptr_t a = (uintptr_t)p;
-  uint32_t s = (uint32_t)(a & 3);
-  const uint32_t* q = (const uint32_t*)(a - s);
+  const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3);
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p - s);
   uint32_t lo = q[0];
   uint32_t hi = (s + (uint32_t)n > 4) ? q[1] : 0u;
   uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, s);
