@@ -80,6 +80,8 @@ def main() -> int:
     ap.add_argument("--values", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -157,6 +159,16 @@ def main() -> int:
         "step_ms_events": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                            "p99": round(float(np.percentile(step_ms, 99)), 4)},
     }
+    if args.copy_calibration:
+        # a plain copy kernel of known size (256 MiB read + 256 MiB written),
+        # whose FETCH_SIZE/WRITE_SIZE scale the decode kernel's counters
+        src = DeviceArray(256 << 20, np.uint8, ctx)
+        dst = DeviceArray(256 << 20, np.uint8, ctx)
+        for _ in range(3):
+            dev_copy(ctx, dst, src, 256 << 20)
+        ctx.sync()
+        src.free()
+        dst.free()
     if not args.no_extras:
         # persistent work-queue form
         e0.record(ctx)

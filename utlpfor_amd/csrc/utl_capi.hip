@@ -80,8 +80,8 @@ int ctx_init(utl_ctx* c, int device) {
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocDefault));
   HIP_TRY(hipMalloc((void**)&c->d_stage, kStageBytes));
-  HIP_TRY(hipMalloc((void**)&c->d_counter, 256));
-  c->tune.grid_wgs = c->cu_count * 8;
+  HIP_TRY(hipMalloc((void**)&c->d_counter, 4096));  // 8 per-XCD work counters, 64 B apart
+  c->tune.grid_wgs = c->cu_count * 4;
   return UTL_OK;
 }
 
@@ -97,9 +97,11 @@ void ctx_free(utl_ctx* c) {
   delete c;
 }
 
+// The per-thread default context lives until the process ends: freeing HIP
+// objects from a thread-exit destructor would run after the HIP runtime (or
+// a profiler layered on it) may already have torn down its own thread state.
 struct TlsCtx {
   utl_ctx* c = nullptr;
-  ~TlsCtx() { ctx_free(c); }
 };
 thread_local TlsCtx g_tls;
 
