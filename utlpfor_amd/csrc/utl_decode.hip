@@ -1,131 +1,137 @@
 // utl_decode.hip -- gfx950 decode kernels for UTL-PFOR blocks.
 //
-// A block is decoded by 16 threads (a quarter of a wavefront, one DPP row);
-// thread L owns UTL lane L, i.e. the 8 values at positions 16r + L. A
-// wavefront decodes 4 blocks side by side and a 256-thread workgroup a tile
-// of 16 consecutive blocks. Tiles are assigned XCD-major (workgroup g runs
-// on XCD g % 8), so each XCD's L2 sees a contiguous range of the stream and
-// cache lines shared by neighbouring blocks are fetched from HBM once.
+// Eight threads decode one block: thread j (0..7) of a block owns the UTL
+// lane pair (2j, 2j+1), i.e. the 16 values at positions 16r + 2j + e. A
+// wavefront decodes 8 consecutive blocks and a 256-thread workgroup a tile
+// of 32. Tiles are assigned XCD-major (workgroup g runs on XCD g % 8), so
+// each XCD's L2 sees one contiguous range of the stream and cache lines
+// shared by neighbouring blocks are fetched from HBM once.
 //
-// Each block is first staged into LDS with 16-byte loads of the aligned
-// chunks that contain its bytes (one round trip to HBM), then decoded from
-// LDS following unpackUint32Scalar (scalar.go:260-309) and its validation
-// order (unpack_block.go:127-237):
+// A wavefront first stages the contiguous bytes of its blocks into an LDS
+// arena with 16-byte loads of the aligned chunks that hold them (one round
+// trip to HBM; groups too large for the arena take further rounds), then
+// decodes every block from LDS following unpackUint32Scalar
+// (scalar.go:260-309) and its validation order (unpack_block.go:127-237):
 //   validate -> unpack (one specialised path per bit width) -> exceptions
-//   (StreamVByte into LDS; bitmap patch by popcount rank, position-list
-//   patch by LDS OR) -> per-lane delta in registers (zigzag, or overflow
-//   checked with the first wrap in v-major order) -> + FOR base.
+//   (StreamVByte values decoded round-robin into LDS; bitmap patch by
+//   popcount rank, position-list patch by LDS OR) -> per-lane delta in
+//   registers (zigzag, or overflow checked, first wrap in v-major order)
+//   -> + FOR base -> 8-byte stores of the lane pairs.
 #include "utl_device.h"
 #include "utl_format.h"
 #include "utl_kernels.h"
 
 namespace utl {
 
-constexpr int kQuad = 16;           // threads per block
-constexpr int kWaveBlocks = 4;      // blocks per wavefront
+constexpr int kTpb = 8;             // threads per block
+constexpr int kWaveBlocks = 8;      // blocks per wavefront
 constexpr int kWgWaves = 4;         // wavefronts per workgroup
 constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 // Bytes of a sub-block the decoder can read: header, svbLen, FOR64 base,
 // block2Len (16) + payload (512) + bitmap (16) + control (32) + data (512).
 constexpr int kSubExtent = 1088;
-constexpr int kSlotBytes = 1120;    // >= 15 + kSubExtent + 4, multiple of 16
+constexpr int kArena = 2560;        // staged bytes per wavefront (>= 15 + 2176 + 4)
 
-struct QuadLds {
-  uint32_t slot[kSlotBytes / 4];    // staged sub-block bytes (at byte offset s)
-  uint32_t aux[kBlock];             // exception high bits, or values (position list)
+struct WaveLds {
+  uint32_t arena[kArena / 4];
+  uint32_t aux[kWaveBlocks][kBlock];  // exception values by rank, or values (position list)
+  uint32_t tab[kWaveBlocks][32];      // per control byte: data offset | control byte << 24
 };
 
-__device__ __forceinline__ void quad_fence() {
+__device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// DPP row (16-lane) primitives. row_shr:n = 0x110 + n, row_ror:n = 0x120 + n.
-template <int Ctrl>
-__device__ __forceinline__ uint32_t dpp(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, Ctrl, 0xF, 0xF, true);
+// DPP row_shr:n within 16-lane rows, masked to 8-lane groups.
+template <int N>
+__device__ __forceinline__ uint32_t grp_shr(uint32_t x, int j) {
+  const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + N, 0xF, 0xF, true);
+  return j >= N ? y : 0u;
 }
 
-__device__ __forceinline__ uint32_t quad_incl_scan(uint32_t x) {
-  x += dpp<0x111>(x);
-  x += dpp<0x112>(x);
-  x += dpp<0x114>(x);
-  x += dpp<0x118>(x);
+__device__ __forceinline__ uint32_t grp_incl_scan(uint32_t x, int j) {
+  x += grp_shr<1>(x, j);
+  x += grp_shr<2>(x, j);
+  x += grp_shr<4>(x, j);
   return x;
 }
 
-__device__ __forceinline__ uint32_t quad_min(uint32_t x) {
-  x = min(x, dpp<0x121>(x));
-  x = min(x, dpp<0x122>(x));
-  x = min(x, dpp<0x124>(x));
-  x = min(x, dpp<0x128>(x));
+__device__ __forceinline__ uint32_t grp_last(uint32_t x) { return (uint32_t)__shfl((int)x, lane_id() | 7, 64); }
+
+__device__ __forceinline__ uint32_t grp_min(uint32_t x) {
+  x = min(x, (uint32_t)__shfl_xor((int)x, 1, 64));
+  x = min(x, (uint32_t)__shfl_xor((int)x, 2, 64));
+  x = min(x, (uint32_t)__shfl_xor((int)x, 4, 64));
   return x;
 }
 
-// Value of lane 15 of this thread's row.
-__device__ __forceinline__ uint32_t quad_last(uint32_t x) { return (uint32_t)__shfl((int)x, (lane_id() & ~15) | 15, 64); }
-
-// n (1..4) little-endian bytes at byte offset off of an LDS slot.
-__device__ __forceinline__ uint32_t lds_bytes(const uint32_t* slot, int off, int n) {
+// n (1..4) little-endian bytes at LDS byte offset off.
+__device__ __forceinline__ uint32_t lds_bytes(const uint32_t* a, int off, int n) {
   const int q = off >> 2, sh = off & 3;
-  const uint32_t lo = slot[q], hi = slot[q + 1];
-  const uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, sh);
+  const uint32_t v = __builtin_amdgcn_alignbyte(a[q + 1], a[q], sh);
   return n >= 4 ? v : (v & ((1u << (8 * n)) - 1u));
 }
-__device__ __forceinline__ uint32_t lds_u32(const uint32_t* slot, int off) { return lds_bytes(slot, off, 4); }
-
-// Stages nbytes at g into the slot: every 16-byte aligned chunk that holds
-// one of the bytes is copied, so the block starts at slot byte (g & 15).
-__device__ __forceinline__ int stage(const uint8_t* __restrict__ g, int64_t nbytes, uint32_t* slot, int L) {
-  const int s = (int)(reinterpret_cast<uintptr_t>(g) & 15);
-  const uint4* base = reinterpret_cast<const uint4*>(g - s);
-  const int nchunks = nbytes > 0 ? (int)((s + nbytes + 15) >> 4) : 0;
-  for (int c = L; c < nchunks; c += kQuad) reinterpret_cast<uint4*>(slot)[c] = base[c];
-  quad_fence();
-  return s;
+__device__ __forceinline__ uint32_t lds_u32(const uint32_t* a, int off) {
+  const int q = off >> 2, sh = off & 3;
+  return __builtin_amdgcn_alignbyte(a[q + 1], a[q], sh);
 }
 
-// UTL unpack of lane L for a fixed bit width (scalar.go:65-116): word w of
-// the lane is at payload byte 64w + 4L; value r occupies bits [r*BW, +BW).
+// UTL unpack of the lane pair (2j, 2j+1) for a fixed bit width
+// (scalar.go:65-116): lane L's word w is at payload byte 64w + 4L; value r
+// occupies bits [r*BW, r*BW+BW) of the lane's bit stream.
 template <int BW>
-__device__ __forceinline__ void unpack_bw(const uint32_t* slot, int pbyte, int L, uint32_t* v) {
+__device__ __forceinline__ void unpack_bw(const uint32_t* a, int pbyte, int j, uint32_t* v0, uint32_t* v1) {
   if constexpr (BW == 0) {
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) v[r] = 0;
+    for (int r = 0; r < kRows; ++r) {
+      v0[r] = 0;  // (separate stores: a chained assignment here sends both arrays to scratch)
+      v1[r] = 0;
+    }
   } else {
     constexpr int NW = BW / 4;
-    uint32_t w[NW + 1];
-    const int base = pbyte + 4 * L;
+    uint32_t w0[NW + 1], w1[NW + 1];
+    const int base = pbyte + 8 * j;
     const int q = base >> 2, sh = base & 3;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) w[k] = __builtin_amdgcn_alignbyte(slot[q + 16 * k + 1], slot[q + 16 * k], sh);
-    w[NW] = 0;
+    for (int k = 0; k < NW; ++k) {
+      const uint32_t d0 = a[q + 16 * k], d1 = a[q + 16 * k + 1], d2 = a[q + 16 * k + 2];
+      w0[k] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+      w1[k] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+    }
+    w0[NW] = w1[NW] = 0;
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
       const int bit = r * BW, wi = bit >> 5, o = bit & 31;
       if constexpr (BW == 32) {
-        v[r] = w[wi];
+        v0[r] = w0[wi];
+        v1[r] = w1[wi];
       } else {
         constexpr uint32_t m = (1u << BW) - 1u;
-        if (o + BW <= 32) v[r] = (w[wi] >> o) & m;
-        else v[r] = __builtin_amdgcn_alignbit(w[wi + 1], w[wi], o) & m;
+        if (o + BW <= 32) {
+          v0[r] = (w0[wi] >> o) & m;
+          v1[r] = (w1[wi] >> o) & m;
+        } else {
+          v0[r] = __builtin_amdgcn_alignbit(w0[wi + 1], w0[wi], o) & m;
+          v1[r] = __builtin_amdgcn_alignbit(w1[wi + 1], w1[wi], o) & m;
+        }
       }
     }
   }
 }
 
-__device__ __forceinline__ void unpack_any(int bw, const uint32_t* slot, int pbyte, int L, uint32_t* v) {
+__device__ __forceinline__ void unpack_any(int bw, const uint32_t* a, int pbyte, int j, uint32_t* v0, uint32_t* v1) {
   switch (bw) {
-    case 0: unpack_bw<0>(slot, pbyte, L, v); break;
-    case 4: unpack_bw<4>(slot, pbyte, L, v); break;
-    case 8: unpack_bw<8>(slot, pbyte, L, v); break;
-    case 12: unpack_bw<12>(slot, pbyte, L, v); break;
-    case 16: unpack_bw<16>(slot, pbyte, L, v); break;
-    case 20: unpack_bw<20>(slot, pbyte, L, v); break;
-    case 24: unpack_bw<24>(slot, pbyte, L, v); break;
-    case 28: unpack_bw<28>(slot, pbyte, L, v); break;
-    default: unpack_bw<32>(slot, pbyte, L, v); break;
+    case 0: unpack_bw<0>(a, pbyte, j, v0, v1); break;
+    case 4: unpack_bw<4>(a, pbyte, j, v0, v1); break;
+    case 8: unpack_bw<8>(a, pbyte, j, v0, v1); break;
+    case 12: unpack_bw<12>(a, pbyte, j, v0, v1); break;
+    case 16: unpack_bw<16>(a, pbyte, j, v0, v1); break;
+    case 20: unpack_bw<20>(a, pbyte, j, v0, v1); break;
+    case 24: unpack_bw<24>(a, pbyte, j, v0, v1); break;
+    case 28: unpack_bw<28>(a, pbyte, j, v0, v1); break;
+    default: unpack_bw<32>(a, pbyte, j, v0, v1); break;
   }
 }
 
@@ -140,20 +146,34 @@ __device__ __forceinline__ int ctrl_bytes(uint32_t c, int n) {
 struct SubResult {
   int count;
   int consumed;
-  bool u64single;
 };
 
-// Decodes one staged sub-block (at slot byte s, true length len) for lane L.
-// ForU64 is the uint64 API's validator and FOR64/combine handling. Returns
-// the status word (code | overflow position << 8).
 template <bool ForU64>
-__device__ int32_t decode_sub(QuadLds* Q, int s, int64_t len, int L, uint32_t* v, SubResult& res) {
-  const uint32_t* slot = Q->slot;
-  res.count = 0;
-  res.consumed = 0;
-  res.u64single = false;
+__device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
+                                                   uint32_t* v1, SubResult& res);
+
+// Decodes one staged sub-block whose first byte is at arena byte `at` and
+// whose true length is len. blk is the wave-local block index (LDS tables),
+// j the thread's index in the block. Returns the status word; count and
+// consumed are zero unless the status is UTL_OK.
+template <bool ForU64>
+__device__ __forceinline__ int32_t decode_sub(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
+                                              uint32_t* v1, SubResult& res) {
+  SubResult r;
+  r.count = 0;
+  r.consumed = 0;
+  const int32_t st = decode_sub_impl<ForU64>(W, blk, at, len, j, v0, v1, r);
+  res.count = st == UTL_OK ? r.count : 0;
+  res.consumed = st == UTL_OK ? r.consumed : 0;
+  return st;
+}
+
+template <bool ForU64>
+__device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
+                                                   uint32_t* v1, SubResult& res) {
+  const uint32_t* a = W->arena;
   if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
-  const Hdr h = decode_header(lds_u32(slot, s));
+  const Hdr h = decode_header(lds_u32(a, at));
   bool combine = h.combine;
   if (ForU64) {
     if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
@@ -182,12 +202,12 @@ __device__ int32_t decode_sub(QuadLds* Q, int s, int64_t len, int L, uint32_t* v
   if (len < end) return UTL_ERR_INVALID_BUFFER;
   int svb_len = 0;
   if (has_exc) {
-    svb_len = (int)lds_bytes(slot, s + kHdr, 2);
+    svb_len = (int)lds_bytes(a, at + kHdr, 2);
     if (end + svb_len > len) return UTL_ERR_INVALID_BUFFER;
   }
-  const uint32_t for_base = (has_for && !u64s) ? lds_bytes(slot, s + for_off, for_base_bytes(h.for_width)) : 0u;
+  const uint32_t for_base = (has_for && !u64s) ? lds_bytes(a, at + for_off, for_base_bytes(h.for_width)) : 0u;
 
-  unpack_any(h.bw, slot, s + poff, L, v);
+  unpack_any(h.bw, a, at + poff, j, v0, v1);
 
   int consumed = poff + pbytes;
   if (has_exc) {
@@ -195,53 +215,77 @@ __device__ int32_t decode_sub(QuadLds* Q, int s, int64_t len, int L, uint32_t* v
     if (h.exc > h.count) return UTL_ERR_INVALID_BUFFER;
     const int ctrl = svb_ctrl_count(h.exc);
     if (ctrl > svb_len) return UTL_ERR_INVALID_BUFFER;
-    const int exc_start = poff + pbytes;
+    const int exc_start = at + poff + pbytes;
     const int svb_start = exc_start + idx;
-    // Thread L owns control bytes 2L, 2L+1, i.e. exceptions 8L .. 8L+7.
-    const int j0 = 2 * L, j1 = 2 * L + 1;
-    const uint32_t cb0 = j0 < ctrl ? lds_bytes(slot, s + svb_start + j0, 1) : 0u;
-    const uint32_t cb1 = j1 < ctrl ? lds_bytes(slot, s + svb_start + j1, 1) : 0u;
-    const int n0 = min(max(h.exc - 4 * j0, 0), 4), n1 = min(max(h.exc - 4 * j1, 0), 4);
-    const int part = ctrl_bytes(cb0, n0) + ctrl_bytes(cb1, n1);
-    const uint32_t incl = quad_incl_scan((uint32_t)part);
-    const int total = (int)quad_last(incl);
-    if (ctrl + total > svb_len) return UTL_ERR_INVALID_BUFFER;
-    // StreamVByte decode of this thread's exceptions into aux[].
-    int off = s + svb_start + ctrl + (int)incl - part;
-    const int nmine = n0 + n1;
+    // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15).
+    const uint32_t cw = 4 * j < ctrl ? lds_u32(a, svb_start + 4 * j) : 0u;
+    int sz[4], part = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k < nmine) {
-        const uint32_t c = k < 4 ? (cb0 >> (2 * k)) : (cb1 >> (2 * (k - 4)));
-        const int l = (int)(c & 3) + 1;
-        Q->aux[8 * L + k] = go_shl32(lds_bytes(slot, off, l), h.bw);
-        off += l;
-      }
+    for (int k = 0; k < 4; ++k) {
+      const int cb = 4 * j + k;
+      const int nslot = min(max(h.exc - 4 * cb, 0), 4);
+      sz[k] = cb < ctrl ? ctrl_bytes((cw >> (8 * k)) & 0xFF, nslot) : 0;
+      part += sz[k];
     }
-    quad_fence();
+    const uint32_t incl = grp_incl_scan((uint32_t)part, j);
+    const int total = (int)grp_last(incl);
+    if (ctrl + total > svb_len) return UTL_ERR_INVALID_BUFFER;
+    // Data offset of each owned control byte, with the byte itself.
+    int off = svb_start + ctrl + (int)incl - part;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      W->tab[blk][4 * j + k] = (uint32_t)off | (((cw >> (8 * k)) & 0xFFu) << 24);
+      off += sz[k];
+    }
+    wave_fence();
+    // StreamVByte values, round-robin: exception i = 8m + j.
+    const int slot = j & 3;
+    const uint32_t pre_mask = (1u << (2 * slot)) - 1u;
+#pragma unroll 1
+    for (int i = j; i < h.exc; i += kTpb) {
+      const uint32_t e = W->tab[blk][i >> 2];
+      const uint32_t c = e >> 24;
+      const uint32_t low = c & pre_mask;
+      const int pre = slot + (int)(low & 3) + (int)((low >> 2) & 3) + (int)((low >> 4) & 3);
+      const int l = (int)((c >> (2 * slot)) & 3) + 1;
+      W->aux[blk][i] = go_shl32(lds_bytes(a, (int)(e & 0xFFFFFF) + pre, l), h.bw);
+    }
+    wave_fence();
     if (h.exc <= kExcBitmap) {
       // Sorted position list: OR each exception into its position in LDS.
       // Full blocks mask positions with 0x7F, partial blocks skip positions
       // >= count (exceptions.go:211-230).
-      const uint32_t hv = L < h.exc ? Q->aux[L] : 0u;
-      quad_fence();
+      const uint32_t hv0 = j < h.exc ? W->aux[blk][j] : 0u;
+      const uint32_t hv1 = j + 8 < h.exc ? W->aux[blk][j + 8] : 0u;
+      wave_fence();
 #pragma unroll
-      for (int r = 0; r < kRows; ++r) Q->aux[16 * r + L] = v[r];
-      quad_fence();
-      if (L < h.exc) {
-        const uint32_t p = lds_bytes(slot, s + exc_start + L, 1);
-        if (h.count == kBlock) atomicOr(&Q->aux[p & 0x7F], hv);
-        else if ((int)p < h.count) atomicOr(&Q->aux[p], hv);
+      for (int r = 0; r < kRows; ++r) {
+        W->aux[blk][16 * r + 2 * j] = v0[r];
+        W->aux[blk][16 * r + 2 * j + 1] = v1[r];
       }
-      quad_fence();
+      wave_fence();
 #pragma unroll
-      for (int r = 0; r < kRows; ++r) v[r] = Q->aux[16 * r + L];
+      for (int e = 0; e < 2; ++e) {
+        const int i = j + 8 * e;
+        if (i < h.exc) {
+          const uint32_t p = lds_bytes(a, exc_start + i, 1);
+          const uint32_t hv = e ? hv1 : hv0;
+          if (h.count == kBlock) atomicOr(&W->aux[blk][p & 0x7F], hv);
+          else if ((int)p < h.count) atomicOr(&W->aux[blk][p], hv);
+        }
+      }
+      wave_fence();
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        v0[r] = W->aux[blk][16 * r + 2 * j];
+        v1[r] = W->aux[blk][16 * r + 2 * j + 1];
+      }
     } else {
       // 128-bit bitmap masked to count; the rank of a set bit is the index
       // of its high bits. popcount(bitmap) > excCount applies nothing.
       uint32_t bm[4];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) bm[d] = lds_u32(slot, s + exc_start + 4 * d);
+      for (int d = 0; d < 4; ++d) bm[d] = lds_u32(a, exc_start + 4 * d);
       if (h.count < kBlock) {
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
@@ -251,146 +295,312 @@ __device__ int32_t decode_sub(QuadLds* Q, int s, int64_t len, int L, uint32_t* v
       }
       const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
       if ((int)(pre3 + __popc(bm[3])) <= h.exc) {
-        const uint32_t m_lo = (1u << L) - 1u, m_hi = (1u << (L + 16)) - 1u;
+        const int b0 = 2 * j;
 #pragma unroll
         for (int r = 0; r < kRows; ++r) {
           const int d = r >> 1;
           const uint32_t word = bm[d];
           const uint32_t pre = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-          const int bit = L + 16 * (r & 1);
-          if ((word >> bit) & 1u) {
-            const uint32_t rank = pre + __popc(word & ((r & 1) ? m_hi : m_lo));
-            v[r] |= Q->aux[rank];
-          }
+          const int bit = b0 + 16 * (r & 1);
+          const uint32_t rank0 = pre + __popc(word & ((1u << bit) - 1u));
+          const uint32_t s0 = (word >> bit) & 1u, s1 = (word >> (bit + 1)) & 1u;
+          const uint32_t x0 = W->aux[blk][rank0 & 127], x1 = W->aux[blk][(rank0 + s0) & 127];
+          v0[r] |= s0 ? x0 : 0u;
+          v1[r] |= s1 ? x1 : 0u;
         }
       }
     }
-    consumed = svb_start + svb_len;
+    consumed = svb_start + svb_len - at;
   }
 
   // ---- per-lane delta decode (delta.go:50-115) ----
   if (h.delta) {
     if (h.zigzag) {
-      v[0] = zz_dec(v[0]);
-#pragma unroll
-      for (int r = 1; r < kRows; ++r) v[r] = v[r - 1] + zz_dec(v[r]);
-    } else {
-      uint32_t first = 0xFFFFFFFFu;
+      v0[0] = zz_dec(v0[0]);
+      v1[0] = zz_dec(v1[0]);
 #pragma unroll
       for (int r = 1; r < kRows; ++r) {
-        const uint32_t nx = v[r - 1] + v[r];
-        const int pos = 16 * r + L;
-        if (nx < v[r - 1] && pos < h.count && first == 0xFFFFFFFFu) first = (uint32_t)pos;
-        v[r] = nx;
+        v0[r] = v0[r - 1] + zz_dec(v0[r]);
+        v1[r] = v1[r - 1] + zz_dec(v1[r]);
       }
-      // v-major order is the flat position: the lowest flagged position wins
-      const uint32_t m = quad_min(first);
-      if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+    } else {
+      uint32_t c0 = 0, c1 = 0;  // rows whose addition wrapped
+#pragma unroll
+      for (int r = 1; r < kRows; ++r) {
+        const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
+        c0 |= (n0 < v0[r - 1] ? 1u : 0u) << r;
+        c1 |= (n1 < v1[r - 1] ? 1u : 0u) << r;
+        v0[r] = n0;
+        v1[r] = n1;
+      }
+      if (__ballot((c0 | c1) != 0)) {
+        // first wrap at a position < count, lowest flat position wins
+        uint32_t first = 0xFFFFFFFFu;
+#pragma unroll
+        for (int r = kRows - 1; r >= 1; --r) {
+          const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
+          if ((c1 >> r) & 1u && (int)p1 < h.count) first = p1;
+          if ((c0 >> r) & 1u && (int)p0 < h.count) first = p0;
+        }
+        const uint32_t m = grp_min(first);
+        if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+      }
     }
   }
   if (has_for && !u64s) {
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) v[r] += for_base;
+    for (int r = 0; r < kRows; ++r) {
+      v0[r] += for_base;
+      v1[r] += for_base;
+    }
   }
   res.count = h.count;
   res.consumed = consumed;
-  res.u64single = u64s;
   return UTL_OK;
 }
 
-__device__ __forceinline__ int64_t cap_extent(int64_t len) { return len < kSubExtent ? len : kSubExtent; }
+// Staged extent of a block: u32 blocks read at most kSubExtent bytes; a
+// uint64 block's two sub-blocks together at most 2 * 1018 + 2 when valid.
+constexpr int kExtent64 = 2176;
 
-// Decodes block b (one quarter-wave) and stores its values.
 template <bool U64>
-__device__ __forceinline__ void decode_block(QuadLds* Q, const uint8_t* __restrict__ src, uint64_t o0, uint64_t o1,
-                                             uint64_t out0, uint64_t slot_n, void* __restrict__ out, int32_t& st,
-                                             int& consumed) {
-  const int L = lane_id() & 15;
-  const uint8_t* buf = src + o0;
-  const int64_t len = (int64_t)(o1 - o0);
-  uint32_t lo[kRows];
-  SubResult r1;
-  int s = stage(buf, cap_extent(len), Q->slot, L);
-  if (!U64) {
-    st = decode_sub<false>(Q, s, len, L, lo, r1);
-    consumed = r1.consumed;
-    if (st == UTL_OK && (uint64_t)r1.count > slot_n) st = UTL_ERR_ARGUMENT;
-    if (st == UTL_OK) {
-      uint32_t* o = (uint32_t*)out + out0;
+__device__ __forceinline__ int64_t clamp_extent(int64_t len) {
+  constexpr int64_t kExt = U64 ? kExtent64 : kSubExtent;
+  return len < 0 ? 0 : (len < kExt ? len : kExt);
+}
+
+// Copies bytes [g, g+n) into the arena at byte (g & 15): every 16-byte
+// aligned chunk that holds one of the bytes, spread over the wave.
+// All of a thread's chunk loads are issued before the first LDS write, so a
+// staged range costs one round trip to HBM.
+__device__ __forceinline__ void stage_range(WaveLds* W, const uint8_t* __restrict__ g, int64_t n) {
+  static_assert((kArena / 16 + 63) / 64 == 3, "three chunks per thread");
+  const int s = (int)(reinterpret_cast<uintptr_t>(g) & 15);
+  const uint4* base = reinterpret_cast<const uint4*>(g - s);
+  const int nchunks = n > 0 ? (int)((s + n + 15) >> 4) : 0;
+  const int l = lane_id();
+  // (scalar variables: a uint4 array here is placed in scratch)
+  uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, c2 = c0;
+  if (l < nchunks) c0 = base[l];
+  if (l + 64 < nchunks) c1 = base[l + 64];
+  if (l + 128 < nchunks) c2 = base[l + 128];
+  uint4* ar = reinterpret_cast<uint4*>(W->arena);
+  if (l < nchunks) ar[l] = c0;
+  if (l + 64 < nchunks) ar[l + 64] = c1;
+  if (l + 128 < nchunks) ar[l + 128] = c2;
+  wave_fence();
+}
+
+__device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t out0, int count, int j,
+                                          const uint32_t* v0, const uint32_t* v1) {
+  uint32_t* o = out + out0 + 2 * j;
+  if (count == kBlock && (out0 & 1) == 0) {
 #pragma unroll
-      for (int r = 0; r < kRows; ++r)
-        if (16 * r + L < r1.count) o[16 * r + L] = lo[r];
-    }
-    return;
-  }
-  // uint64.go:191-266 unpackUint64Block
-  consumed = 0;
-  if (len < kHdr) {
-    st = UTL_ERR_INVALID_BUFFER;
-    return;
-  }
-  const Hdr h = decode_header(lds_u32(Q->slot, s));
-  if (h.int_type != 3) {
-    st = UTL_ERR_UNSUPPORTED_TYPE;
-    return;
-  }
-  uint64_t y[kRows];
-  int count;
-  if (!h.combine) {
-    uint64_t base = 0;
-    if (is_for64_single(h.int_type, h.for_width, h.combine)) {
-      const int boff = kHdr + (h.exc > 0 ? kSvbLen : 0);
-      if (len < boff + kFor64) {
-        st = UTL_ERR_INVALID_BUFFER;
-        return;
-      }
-      base = (uint64_t)lds_u32(Q->slot, s + boff) | ((uint64_t)lds_u32(Q->slot, s + boff + 4) << 32);
-    }
-    st = decode_sub<true>(Q, s, len, L, lo, r1);
-    if (st) return;
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) y[r] = (uint64_t)lo[r] + base;
-    count = r1.count;
-    consumed = r1.consumed;
+    for (int r = 0; r < kRows; ++r) *reinterpret_cast<uint2*>(o + 16 * r) = make_uint2(v0[r], v1[r]);
   } else {
-    st = decode_sub<true>(Q, s, len, L, lo, r1);
-    if (st) return;
-    const bool has_exc = h.exc > 0;
-    const int forb = for_base_bytes(h.for_width);
-    const int meta = payload_offset(forb, has_exc, true);
-    if (len < meta) {
-      st = UTL_ERR_INVALID_BUFFER;
-      return;
-    }
-    const int b2len = (int)lds_bytes(Q->slot, s + kHdr + (has_exc ? kSvbLen : 0) + forb, 2);
-    const int64_t b2s = r1.consumed;
-    if (b2s + b2len > len) {
-      st = UTL_ERR_INVALID_BUFFER;
-      return;
-    }
-    quad_fence();
-    uint32_t hi[kRows];
-    SubResult r2;
-    s = stage(buf + b2s, cap_extent(b2len), Q->slot, L);
-    st = decode_sub<true>(Q, s, b2len, L, hi, r2);
-    if (st) return;
-    if (r2.count != r1.count) {
-      st = UTL_ERR_INVALID_BUFFER;
-      return;
-    }
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) y[r] = ((uint64_t)hi[r] << 32) | lo[r];
+    for (int r = 0; r < kRows; ++r) {
+      const int p = 16 * r + 2 * j;
+      if (p < count) o[16 * r] = v0[r];
+      if (p + 1 < count) o[16 * r + 1] = v1[r];
+    }
+  }
+}
+
+// two: values are (hi << 32) | lo; else lo + base.
+__device__ __forceinline__ void store_u64(uint64_t* __restrict__ out, uint64_t out0, int count, int j,
+                                          const uint32_t* l0, const uint32_t* l1, const uint32_t* h0,
+                                          const uint32_t* h1, bool two, uint64_t base) {
+  uint64_t* o = out + out0 + 2 * j;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const uint64_t y0 = two ? (((uint64_t)h0[r] << 32) | l0[r]) : (uint64_t)l0[r] + base;
+    const uint64_t y1 = two ? (((uint64_t)h1[r] << 32) | l1[r]) : (uint64_t)l1[r] + base;
+    const int p = 16 * r + 2 * j;
+    if (count == kBlock && (out0 & 1) == 0) {
+      ulonglong2 q;
+      q.x = y0;
+      q.y = y1;
+      *reinterpret_cast<ulonglong2*>(o + 16 * r) = q;
+    } else {
+      if (p < count) o[16 * r] = y0;
+      if (p + 1 < count) o[16 * r + 1] = y1;
+    }
+  }
+}
+
+// State of one uint64 two-block block between its sub-block decodes.
+struct U64Pending {
+  int c1, b2len;  // block1Consumed and block2Len
+  bool pending;   // sub-block 2 lies outside the staged extent
+};
+
+// uint64.go:191-266 unpackUint64Block, first part: everything up to the
+// second sub-block of a two-block block. Returns the status; on a two-block
+// block whose sub-block 2 is staged it is decoded here too.
+__device__ __forceinline__ int32_t decode_u64_block(WaveLds* W, int blk, int at, int64_t len, int64_t avail, int j, uint32_t* l0,
+                                    uint32_t* l1, uint32_t* h0, uint32_t* h1, uint64_t& base, bool& two, int& count,
+                                    int& consumed, U64Pending& pend) {
+  two = false;
+  base = 0;
+  count = 0;
+  consumed = 0;
+  pend.pending = false;
+  if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
+  const Hdr h = decode_header(lds_u32(W->arena, at));
+  if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
+  SubResult r1;
+  if (!h.combine) {
+    if (is_for64_single(h.int_type, h.for_width, h.combine)) {
+      const int bo = kHdr + (h.exc > 0 ? kSvbLen : 0);
+      if (len < bo + kFor64) return UTL_ERR_INVALID_BUFFER;
+      base = (uint64_t)lds_u32(W->arena, at + bo) | ((uint64_t)lds_u32(W->arena, at + bo + 4) << 32);
+    }
+    const int32_t st = decode_sub<true>(W, blk, at, len, j, l0, l1, r1);
     count = r1.count;
-    consumed = r1.consumed + b2len;
+    consumed = r1.consumed;
+    return st;
   }
-  if ((uint64_t)count > slot_n) {
-    st = UTL_ERR_ARGUMENT;
-    return;
+  int32_t st = decode_sub<true>(W, blk, at, len, j, l0, l1, r1);
+  if (st) return st;
+  const bool he = h.exc > 0;
+  const int forb = for_base_bytes(h.for_width);
+  if (len < payload_offset(forb, he, true)) return UTL_ERR_INVALID_BUFFER;
+  const int b2len = (int)lds_bytes(W->arena, at + kHdr + (he ? kSvbLen : 0) + forb, 2);
+  const int64_t b2s = r1.consumed;
+  if (b2s + b2len > len) return UTL_ERR_INVALID_BUFFER;
+  two = true;
+  count = r1.count;
+  pend.c1 = r1.consumed;
+  pend.b2len = b2len;
+  if (b2s + clamp_extent<false>(b2len) > avail) {
+    pend.pending = true;
+    return UTL_OK;
   }
-  uint64_t* o = (uint64_t*)out + out0;
+  SubResult r2;
+  st = decode_sub<true>(W, blk, at + (int)b2s, b2len, j, h0, h1, r2);
+  if (st) return st;
+  if (r2.count != r1.count) return UTL_ERR_INVALID_BUFFER;
+  consumed = r1.consumed + b2len;
+  return UTL_OK;
+}
+
+// Decodes the wave's blocks [b0, b0 + nb). The wave-uniform round logic
+// stages as many consecutive blocks as fit the arena, decodes them, and
+// repeats until every block is done.
+template <bool U64>
+__device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
+                             const uint64_t* __restrict__ ooff, uint64_t b0, int nb, void* __restrict__ out,
+                             int32_t* __restrict__ status, int32_t* __restrict__ consumed_out) {
+  const int blk = lane_id() >> 3, j = lane_id() & 7;
+  const bool mine = blk < nb;
+  const uint64_t b = b0 + (mine ? blk : 0);
+  const uint64_t my_o0 = boff[b], my_o1 = boff[b + 1];
+  const uint64_t out0 = ooff[b], slot_n = ooff[b + 1] - out0;
+  const int64_t len = (int64_t)(my_o1 - my_o0);
+  // Common case: the whole group fits the arena in one round. Its offsets
+  // are loaded together (uniform loads) and checked without a loop.
+  uint64_t o[kWaveBlocks + 1];
 #pragma unroll
-  for (int r = 0; r < kRows; ++r)
-    if (16 * r + L < count) o[16 * r + L] = y[r];
+  for (int k = 0; k <= kWaveBlocks; ++k) o[k] = boff[b0 + (k < nb ? k : nb)];
+  bool mono = true;
+  uint64_t o_lastm1 = o[0];
+#pragma unroll
+  for (int k = 0; k < kWaveBlocks; ++k) {
+    if (k < nb) mono = mono && o[k + 1] >= o[k];
+    if (k == nb - 1) o_lastm1 = o[k];
+  }
+  const int64_t ext_all = clamp_extent<U64>((int64_t)(o[kWaveBlocks] - o_lastm1));
+  const bool one_round = mono && (int64_t)(o_lastm1 - o[0]) + ext_all + 15 + 4 <= kArena;
+  int first = 0;
+#pragma unroll 1
+  while (first < nb) {
+    uint64_t base, lo;
+    int last;
+    int64_t ext_last;
+    if (one_round) {
+      base = o[0];
+      last = nb;
+      lo = o_lastm1;
+      ext_last = ext_all;
+    } else {
+      base = boff[b0 + first];
+      last = first;
+#pragma unroll 1
+      while (last < nb) {
+        const uint64_t o0 = boff[b0 + last], o1 = boff[b0 + last + 1];
+        const int64_t ext = clamp_extent<U64>((int64_t)(o1 - o0));
+        if (o0 < base || (int64_t)(o0 - base) + ext + 15 + 4 > kArena) break;
+        ++last;
+      }
+      if (last == first) last = first + 1;
+      lo = boff[b0 + last - 1];
+      ext_last = clamp_extent<U64>((int64_t)(boff[b0 + last] - lo));
+    }
+    const int64_t span = (last - first == 1) ? ext_last : (int64_t)(lo - base) + ext_last;
+    const uint8_t* gbase = src + base;
+    stage_range(W, gbase, span);
+    int jj = j, bb = blk;  // opaque per round: keeps per-thread LDS addresses out of the round loop
+    asm volatile("" : "+v"(jj), "+v"(bb));
+    const bool active = mine && bb >= first && bb < last;
+    const int at = (int)(reinterpret_cast<uintptr_t>(gbase) & 15) + (int)(my_o0 - base);
+    int32_t st = UTL_OK;
+    int consumed = 0;
+    if (!U64) {
+      if (active) {
+        uint32_t v0[kRows], v1[kRows];
+        SubResult r1;
+        st = decode_sub<false>(W, bb, at, len, jj, v0, v1, r1);
+        consumed = r1.consumed;
+        if (st == UTL_OK && (uint64_t)r1.count > slot_n) st = UTL_ERR_ARGUMENT;
+        if (st == UTL_OK && r1.count > 0) store_u32((uint32_t*)out, out0, r1.count, jj, v0, v1);
+        if (jj == 0) {
+          status[b] = st;
+          if (consumed_out) consumed_out[b] = st == UTL_OK ? consumed : 0;
+        }
+      }
+      wave_fence();
+    } else {
+      uint32_t l0[kRows], l1[kRows], h0[kRows], h1[kRows];
+      uint64_t base64 = 0;
+      bool two = false;
+      int count = 0;
+      U64Pending pend;
+      pend.pending = false;
+      if (active)
+        st = decode_u64_block(W, bb, at, len, clamp_extent<true>(len), jj, l0, l1, h0, h1, base64, two, count,
+                              consumed, pend);
+      wave_fence();
+      // Corrupt blocks only: sub-block 2 outside the staged extent gets a
+      // staged round of its own, one block at a time.
+      uint64_t pmask = __ballot(active && st == UTL_OK && pend.pending && j == 0);
+      while (pmask) {
+        const int k = __ffsll((long long)pmask) - 1;  // lane of the block's thread 0
+        pmask &= pmask - 1;
+        const uint64_t kb = b0 + (k >> 3);
+        const int c1 = __shfl(pend.c1, k, 64), b2len = __shfl(pend.b2len, k, 64);
+        const uint8_t* g = src + boff[kb] + c1;
+        stage_range(W, g, clamp_extent<false>(b2len));
+        if (bb == (k >> 3)) {
+          SubResult r2;
+          st = decode_sub<true>(W, bb, (int)(reinterpret_cast<uintptr_t>(g) & 15), b2len, jj, h0, h1, r2);
+          if (st == UTL_OK && r2.count != count) st = UTL_ERR_INVALID_BUFFER;
+          if (st == UTL_OK) consumed = c1 + b2len;
+        }
+        wave_fence();
+      }
+      if (active) {
+        if (st == UTL_OK && (uint64_t)count > slot_n) st = UTL_ERR_ARGUMENT;
+        if (st == UTL_OK && count > 0)
+          store_u64((uint64_t*)out, out0, count, jj, l0, l1, h0, h1, two, base64);
+        if (jj == 0) {
+          status[b] = st;
+          if (consumed_out) consumed_out[b] = st == UTL_OK ? consumed : 0;
+        }
+      }
+    }
+    first = last;
+  }
 }
 
 // Workgroup g runs on XCD g % 8; tile(g) makes each XCD's workgroups cover
@@ -398,37 +608,21 @@ __device__ __forceinline__ void decode_block(QuadLds* Q, const uint8_t* __restri
 __device__ __forceinline__ uint64_t xcd_tile(uint64_t g, uint64_t per_xcd) { return (g & 7) * per_xcd + (g >> 3); }
 
 template <bool U64>
-__device__ __forceinline__ void decode_tile_block(QuadLds* Q, const uint8_t* __restrict__ src,
-                                                  const uint64_t* __restrict__ boff, const uint64_t* __restrict__ ooff,
-                                                  uint64_t b, void* __restrict__ out, int32_t* __restrict__ status,
-                                                  int32_t* __restrict__ consumed_out) {
-  const uint64_t o0 = boff[b], o1 = boff[b + 1];
-  const uint64_t out0 = ooff[b], out1 = ooff[b + 1];
-  int32_t st;
-  int consumed;
-  decode_block<U64>(Q, src, o0, o1, out0, out1 - out0, out, st, consumed);
-  if ((lane_id() & 15) == 0) {
-    status[b] = st;
-    if (consumed_out) consumed_out[b] = st == UTL_OK ? consumed : 0;
-  }
-}
-
-// Flat grid: one tile of 16 blocks per workgroup, XCD-major.
-template <bool U64>
 __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                                 const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                 void* __restrict__ out, int32_t* __restrict__ status,
                                                 int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
-  __shared__ QuadLds lds[kTileBlocks];
+  __shared__ WaveLds lds[kWgWaves];
+  const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t tile = xcd_tile(blockIdx.x, per_xcd);
-  const int quad = (int)(threadIdx.x >> 4);
-  const uint64_t b = tile * kTileBlocks + quad;
-  if (b >= nblocks) return;  // whole quarter exits together
-  decode_tile_block<U64>(&lds[quad], src, boff, ooff, b, out, status, consumed_out);
+  const uint64_t b0 = tile * kTileBlocks + (uint64_t)wave * kWaveBlocks;
+  if (b0 >= nblocks) return;
+  const int nb = (int)min((uint64_t)kWaveBlocks, nblocks - b0);
+  decode_group<U64>(&lds[wave], src, boff, ooff, b0, nb, out, status, consumed_out);
 }
 
 // Persistent form: a fixed grid; workgroups on XCD x take tiles from XCD x's
-// own work counter (counter[x]) over XCD x's contiguous range of tiles, so
+// own work counter (counter[8x]) over XCD x's contiguous range of tiles, so
 // load balance stays inside the XCD. Every workgroup reaches the exit test.
 template <bool U64>
 __global__ __launch_bounds__(256) void k_decode_persistent(const uint8_t* __restrict__ src,
@@ -438,10 +632,10 @@ __global__ __launch_bounds__(256) void k_decode_persistent(const uint8_t* __rest
                                                            int32_t* __restrict__ consumed_out,
                                                            unsigned long long* __restrict__ counter, uint64_t per_xcd,
                                                            uint32_t tiles_per_grab) {
-  __shared__ QuadLds lds[kTileBlocks];
+  __shared__ WaveLds lds[kWgWaves];
   __shared__ unsigned long long grab;
   const uint32_t xcd = blockIdx.x & 7;
-  const int quad = (int)(threadIdx.x >> 4);
+  const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t ntiles = (nblocks + kTileBlocks - 1) / kTileBlocks;
   const uint64_t first = xcd * per_xcd;
   const uint64_t last = min(first + per_xcd, ntiles);
@@ -453,9 +647,11 @@ __global__ __launch_bounds__(256) void k_decode_persistent(const uint8_t* __rest
     if (t0 >= last) break;
     const uint64_t t1 = min(t0 + tiles_per_grab, last);
     for (uint64_t tile = t0; tile < t1; ++tile) {
-      const uint64_t b = tile * kTileBlocks + quad;
-      if (b < nblocks) decode_tile_block<U64>(&lds[quad], src, boff, ooff, b, out, status, consumed_out);
-      quad_fence();
+      const uint64_t b0 = tile * kTileBlocks + (uint64_t)wave * kWaveBlocks;
+      if (b0 < nblocks) {
+        const int nb = (int)min((uint64_t)kWaveBlocks, nblocks - b0);
+        decode_group<U64>(&lds[wave], src, boff, ooff, b0, nb, out, status, consumed_out);
+      }
     }
   }
 }
