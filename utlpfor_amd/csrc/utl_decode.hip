@@ -32,7 +32,7 @@ constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 constexpr int kSubExtent = 1088;
 constexpr int kArena = 2560;        // staged bytes per wavefront (>= 15 + 2176 + 4)
 
-struct WaveLds {
+struct alignas(16) WaveLds {
   uint32_t arena[kArena / 4];
   uint32_t aux[kWaveBlocks][kBlock];  // exception values by rank, or values (position list)
   uint32_t tab[kWaveBlocks][32];      // per control byte: data offset | control byte << 24
@@ -135,14 +135,6 @@ __device__ __forceinline__ void unpack_any(int bw, const uint32_t* a, int pbyte,
   }
 }
 
-// Data bytes described by control byte c when only its first n slots count.
-__device__ __forceinline__ int ctrl_bytes(uint32_t c, int n) {
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) s += (k < n) ? (int)((c >> (2 * k)) & 3) + 1 : 0;
-  return s;
-}
-
 struct SubResult {
   int count;
   int consumed;
@@ -218,25 +210,27 @@ __device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, 
     const int exc_start = at + poff + pbytes;
     const int svb_start = exc_start + idx;
     // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15).
-    const uint32_t cw = 4 * j < ctrl ? lds_u32(a, svb_start + 4 * j) : 0u;
-    int sz[4], part = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int cb = 4 * j + k;
-      const int nslot = min(max(h.exc - 4 * cb, 0), 4);
-      sz[k] = cb < ctrl ? ctrl_bytes((cw >> (8 * k)) & 0xFF, nslot) : 0;
-      part += sz[k];
-    }
+    // Only the slots of exceptions < excCount count (exceptions.go:63-80):
+    // data bytes = valid slots + sum of their 2-bit codes, summed per byte
+    // with SWAR arithmetic.
+    const int nv = min(max(h.exc - 16 * j, 0), 16);  // valid slots in this dword
+    const uint32_t cw = nv > 0 ? lds_u32(a, svb_start + 4 * j) : 0u;
+    const uint32_t cm = cw & (nv >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nv)) - 1u));
+    const uint32_t t2 = (cm & 0x33333333u) + ((cm >> 2) & 0x33333333u);
+    const uint32_t t4 = (t2 & 0x0F0F0F0Fu) + ((t2 >> 4) & 0x0F0F0F0Fu);  // code sum per byte
+    const uint32_t pre = t4 * 0x01010100u;                                // exclusive byte prefix
+    const int part = nv + (int)((t4 * 0x01010101u) >> 24);
     const uint32_t incl = grp_incl_scan((uint32_t)part, j);
     const int total = (int)grp_last(incl);
     if (ctrl + total > svb_len) return UTL_ERR_INVALID_BUFFER;
     // Data offset of each owned control byte, with the byte itself.
-    int off = svb_start + ctrl + (int)incl - part;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      W->tab[blk][4 * j + k] = (uint32_t)off | (((cw >> (8 * k)) & 0xFFu) << 24);
-      off += sz[k];
-    }
+    const int off = svb_start + ctrl + (int)incl - part;
+    uint4 te;
+    te.x = (uint32_t)(off + ((pre >> 0) & 0xFF)) | ((cw & 0xFFu) << 24);
+    te.y = (uint32_t)(off + min(4, nv) + ((pre >> 8) & 0xFF)) | (((cw >> 8) & 0xFFu) << 24);
+    te.z = (uint32_t)(off + min(8, nv) + ((pre >> 16) & 0xFF)) | (((cw >> 16) & 0xFFu) << 24);
+    te.w = (uint32_t)(off + min(12, nv) + (pre >> 24)) | ((cw >> 24) << 24);
+    *reinterpret_cast<uint4*>(&W->tab[blk][4 * j]) = te;
     wave_fence();
     // StreamVByte values, round-robin: exception i = 8m + j.
     const int slot = j & 3;
@@ -324,23 +318,23 @@ __device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, 
         v1[r] = v1[r - 1] + zz_dec(v1[r]);
       }
     } else {
-      uint32_t c0 = 0, c1 = 0;  // rows whose addition wrapped
+      uint64_t any = 0;  // lanes of the wave with a wrapped addition
 #pragma unroll
       for (int r = 1; r < kRows; ++r) {
         const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
-        c0 |= (n0 < v0[r - 1] ? 1u : 0u) << r;
-        c1 |= (n1 < v1[r - 1] ? 1u : 0u) << r;
+        any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
         v0[r] = n0;
         v1[r] = n1;
       }
-      if (__ballot((c0 | c1) != 0)) {
-        // first wrap at a position < count, lowest flat position wins
+      if (any) {
+        // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
+        // first wrap at a position < count in v-major order wins.
         uint32_t first = 0xFFFFFFFFu;
 #pragma unroll
         for (int r = kRows - 1; r >= 1; --r) {
           const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
-          if ((c1 >> r) & 1u && (int)p1 < h.count) first = p1;
-          if ((c0 >> r) & 1u && (int)p0 < h.count) first = p0;
+          if (v1[r] < v1[r - 1] && (int)p1 < h.count) first = p1;
+          if (v0[r] < v0[r - 1] && (int)p0 < h.count) first = p0;
         }
         const uint32_t m = grp_min(first);
         if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
