@@ -80,6 +80,9 @@ def main() -> int:
     ap.add_argument("--values", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step (for profiling)")
+    ap.add_argument("--workload", default="delta64m", choices=["delta64m", "plain1m", "uneven", "u64"],
+                    help="delta64m is the headline; the others print their own JSON lines")
+    ap.add_argument("--streams", type=int, default=200_000, help="streams of the uneven workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -93,6 +96,16 @@ def main() -> int:
     ctx = Context(0)
     name, cus = ctx.device_info()
     log(f"device {name}, {cus} CUs")
+    if args.workload != "delta64m":
+        from utlpfor_amd import runbench
+        if args.workload == "plain1m":
+            print(json.dumps(runbench.plain_1m(ctx)), flush=True)
+        elif args.workload == "uneven":
+            print(json.dumps(runbench.uneven(ctx, nstreams=args.streams)), flush=True)
+        elif args.workload == "u64":
+            for rec in runbench.u64_and_get(ctx):
+                print(json.dumps(rec), flush=True)
+        return 0
     n = args.values
     log(f"generating {n} delta postings")
     v = workloads.delta_postings(n)

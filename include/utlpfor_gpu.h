@@ -194,6 +194,18 @@ int utl_compact(utl_ctx *ctx, int is_u64, const uint8_t *d_slots, const uint64_t
 int utl_make_jobs(utl_ctx *ctx, const uint64_t *d_val_off, const uint64_t *d_blk_off, const uint8_t *d_flags,
                   size_t nstreams, uint64_t *d_in_off, uint32_t *d_in_meta);
 
+/* One resident stream's whole decode -- the device index scan over d_meta
+ * and the batched decode (persistent work-queue form when `persistent`, with
+ * the context's current launch shape) -- captured once as a HIP graph and
+ * replayed by utl_graph_launch on the context's stream. All pointers are
+ * fixed at capture. Returns NULL on failure (utl_last_error). */
+typedef struct utl_graph utl_graph;
+utl_graph *utl_graph_decode_create(utl_ctx *ctx, int is_u64, int persistent, const uint8_t *d_src,
+                                   const uint32_t *d_meta, size_t nblocks, uint64_t *d_block_off,
+                                   uint64_t *d_out_off, void *d_out, int32_t *d_status);
+int utl_graph_launch(utl_graph *g);
+void utl_graph_destroy(utl_graph *g);
+
 /* HIP events on the context's stream, for timing device work in place. */
 int utl_event_create(void **ev);
 int utl_event_destroy(void *ev);

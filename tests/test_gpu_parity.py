@@ -328,6 +328,48 @@ def test_batched_mixed_streams_and_corruption(gpu, orc):
     cs.free()
 
 
+def test_uneven_streams_graph_autotune(gpu, orc):
+    """Mixed-flag uneven streams: GPU encode bytes equal the oracle's stream
+    by stream; the autotuned persistent decode captured as a HIP graph equals
+    the flat decode on every replay (eager work and syncs between replays)."""
+    from utlpfor_amd import autotune, batch, workloads
+    from utlpfor_amd.device import DeviceArray
+    codec, ctx = gpu
+    v, lens, flags = workloads.uneven_streams(300, max_len=20000, seed=11)
+    say(f"uneven: {lens.size} streams, {v.size} values")
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    got = s.bytes_host()
+    off = 0
+    pos = 0
+    for L, f in zip(lens, flags):
+        want = orc.pack_stream(int(f), v[pos:pos + int(L)])[0]
+        assert np.array_equal(got[off:off + want.size], want), (int(L), int(f))
+        off += want.size
+        pos += int(L)
+    assert off == s.nbytes
+    flat_out, flat_st = batch.decode(ctx, s)
+    ctx.sync()
+    ref = flat_out.to_numpy()
+    assert np.all(flat_st.to_numpy() == 0) and np.array_equal(ref, v)
+    tune = autotune.tune_decode(ctx, s, use_cache=False, reps=2)
+    assert tune["grid_wgs"] % 8 == 0 and tune["tiles_per_grab"] >= 1
+    out = DeviceArray(v.size, np.uint32, ctx)
+    st = DeviceArray(s.nblocks, np.int32, ctx)
+    g = batch.DecodeGraph(ctx, s, out, st, persistent=True)
+    for rep in range(3):
+        out.fill_bytes(0xA5)
+        ctx.sync()
+        g.launch()
+        ctx.sync()
+        assert np.all(st.to_numpy() == 0), rep
+        assert np.array_equal(out.to_numpy(), ref), rep
+        # eager work between replays
+        batch.decode(ctx, s, out=flat_out, status=flat_st)
+        ctx.sync()
+    g.close()
+    s.free()
+
+
 def test_batched_u64(gpu, orc):
     from utlpfor_amd import batch
     from utlpfor_amd.device import DeviceArray

@@ -54,6 +54,9 @@ SIGNATURES = {
     "utl_encode_blocks": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp]),
     "utl_compact": (C.c_int, [vp, C.c_int, vp, vp, sz, vp]),
     "utl_make_jobs": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),
+    "utl_graph_decode_create": (vp, [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, vp]),
+    "utl_graph_launch": (C.c_int, [vp]),
+    "utl_graph_destroy": (None, [vp]),
     "utl_event_create": (C.c_int, [C.POINTER(vp)]),
     "utl_event_destroy": (C.c_int, [vp]),
     "utl_event_record": (C.c_int, [vp, vp]),

@@ -146,6 +146,34 @@ def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: Device
     return out, status
 
 
+class DecodeGraph:
+    """The whole decode of a resident stream (device index scan + decode)
+    captured once as a HIP graph and replayed with one launch."""
+
+    def __init__(self, ctx: Context, s: DeviceStream, out: DeviceArray, status: DeviceArray,
+                 persistent: bool = True):
+        lib = _lib.load()
+        self.ctx = ctx
+        self.handle = lib.utl_graph_decode_create(ctx.handle, int(s.is64), int(persistent), s.data.ptr, s.meta.ptr,
+                                                  s.nblocks, s.block_off.ptr, s.out_off.ptr, out.ptr, status.ptr)
+        if not self.handle:
+            raise RuntimeError(f"graph capture failed: {_lib.last_error()}")
+
+    def launch(self) -> None:
+        check(_lib.load().utl_graph_launch(self.handle), "utl_graph_launch")
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().utl_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def set_tune(ctx: Context, waves_per_wg: int, grid_wgs: int, blocks_per_wave: int) -> None:
     check(_lib.load().utl_set_tune(ctx.handle, waves_per_wg, grid_wgs, blocks_per_wave), "utl_set_tune")
 

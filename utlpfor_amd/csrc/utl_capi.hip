@@ -541,6 +541,72 @@ int utl_make_jobs(utl_ctx* ctx, const uint64_t* d_val_off, const uint64_t* d_blk
   return UTL_OK;
 }
 
+// ---- HIP graph of index scan + decode ----
+
+}  // extern "C"
+
+struct utl_graph {
+  utl_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+utl_graph* utl_graph_decode_create(utl_ctx* ctx, int is_u64, int persistent, const uint8_t* d_src,
+                                   const uint32_t* d_meta, size_t nblocks, uint64_t* d_block_off,
+                                   uint64_t* d_out_off, void* d_out, int32_t* d_status) {
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c || !d_meta || !d_block_off || !d_out_off || (nblocks && (!d_src || !d_out || !d_status))) return nullptr;
+  // everything the captured kernels touch must exist before capture
+  if (ensure_tmp(c, utl_index_tmp_bytes(nblocks))) return nullptr;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+  utl_graph* g = new utl_graph();
+  g->ctx = c;
+  bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    ok = utl_launch_index(d_meta, nblocks, d_block_off, d_out_off, c->d_tmp, c->stream) == hipSuccess;
+    if (ok) {
+      if (persistent)
+        ok = utl_launch_decode_persistent(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_out, d_status,
+                                          nullptr, c->d_counter, c->tune, c->stream) == hipSuccess;
+      else
+        ok = utl_launch_decode(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, nullptr,
+                               c->stream) == hipSuccess;
+    }
+    hipGraph_t graph = nullptr;
+    const bool ended = hipStreamEndCapture(c->stream, &graph) == hipSuccess;
+    ok = ok && ended && graph;
+    g->graph = graph;
+  }
+  if (ok) ok = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) == hipSuccess;
+  if (!ok) {
+    g_last_error = "graph capture failed";
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return nullptr;
+  }
+  return g;
+}
+
+int utl_graph_launch(utl_graph* g) {
+  if (!g) return UTL_ERR_ARGUMENT;
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream));
+  return UTL_OK;
+}
+
+void utl_graph_destroy(utl_graph* g) {
+  if (!g) return;
+  hipSetDevice(g->ctx->device);
+  hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+}
+
 // ---- timing and calibration helpers ----
 
 int utl_event_create(void** ev) {

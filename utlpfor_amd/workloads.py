@@ -37,26 +37,41 @@ def delta_postings(n: int = 64 << 20, seed: int = 2) -> np.ndarray:
     return (np.cumsum(gaps, dtype=np.uint64) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
 
 
-def uneven_streams(nstreams: int = 200_000, max_len: int = 200_000, seed: int = 3):
-    """Returns (values uint32, stream lengths, stream flags)."""
+def uneven_streams(nstreams: int = 200_000, max_len: int = 200_000, seed: int = 3, progress=None):
+    """Returns (values uint32, stream lengths, stream flags). Generated in
+    chunks of streams with vectorised numpy (segmented cumulative sums for
+    the random walks)."""
     rng = np.random.default_rng(seed)
     lens = np.floor(np.exp(rng.uniform(0, np.log(max_len + 1), nstreams))).astype(np.int64)
     lens = np.clip(lens, 1, max_len)
     flags = STREAM_FLAGS[rng.integers(0, STREAM_FLAGS.size, nstreams)]
-    total = int(lens.sum())
-    out = np.empty(total, dtype=np.uint32)
     walk = rng.random(nstreams) < 0.5
     bits = rng.integers(0, 33, nstreams)
-    off = 0
-    for s in range(nstreams):
-        L = int(lens[s])
-        if walk[s]:
-            steps = rng.integers(-(1 << 10), (1 << 10) + 1, L)
-            out[off:off + L] = (np.cumsum(steps) + int(rng.integers(0, 1 << 31))).astype(np.int64).astype(np.uint32)
-        else:
-            out[off:off + L] = rng.integers(0, 1 << int(bits[s]), L, dtype=np.uint64).astype(np.uint32) \
-                if bits[s] > 0 else 0
-        off += L
+    start = rng.integers(0, 1 << 31, nstreams)
+    ends = np.cumsum(lens)
+    total = int(ends[-1])
+    out = np.empty(total, dtype=np.uint32)
+    chunk = 1 << 26
+    s0 = 0
+    while s0 < nstreams:
+        a = int(ends[s0 - 1]) if s0 else 0
+        s1 = int(np.searchsorted(ends, a + chunk, side="right"))
+        s1 = max(s1, s0 + 1)
+        b = int(ends[s1 - 1])
+        L = lens[s0:s1]
+        sid = np.repeat(np.arange(s0, s1), L)
+        # uniform in [0, 2^b): the top b bits of a uniform 32-bit draw
+        raw = rng.integers(0, 1 << 32, b - a, dtype=np.uint64)
+        uni = raw >> (np.uint64(32) - bits[sid].astype(np.uint64))
+        steps = rng.integers(-(1 << 10), (1 << 10) + 1, b - a, dtype=np.int32).astype(np.int64)
+        cs = np.cumsum(steps)
+        seg0 = np.concatenate([[0], np.cumsum(L)[:-1]])
+        base = np.repeat(cs[seg0] - steps[seg0], L)  # cumsum before each stream's first step
+        wv = (start[sid] + cs - base).astype(np.int64).astype(np.uint32)
+        out[a:b] = np.where(walk[sid], wv, uni.astype(np.uint32))
+        if progress:
+            progress(f"generated {b}/{total} values")
+        s0 = s1
     return out, lens.astype(np.uint64), flags
 
 
