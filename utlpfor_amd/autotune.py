@@ -1,7 +1,7 @@
 """Autotuner for the persistent decode launch.
 
 The persistent decode kernel runs a fixed grid whose workgroups pull tiles
-of 32 blocks from per-XCD work counters. Two knobs shape it: the grid size
+of 32 blocks (four wavefronts of eight) from per-XCD work counters. Two knobs shape it: the grid size
 (workgroups, a multiple of 8 so every XCD gets the same count) and the tiles
 taken per grab (a run of consecutive tiles per atomic). The tuner times
 every candidate on the caller's own resident stream with HIP events, keeps

@@ -2,8 +2,8 @@
 //
 // Eight threads decode one block: thread j (0..7) of a block owns the UTL
 // lane pair (2j, 2j+1), i.e. the 16 values at positions 16r + 2j + e. A
-// wavefront decodes 8 consecutive blocks and a 256-thread workgroup a tile
-// of 32. Tiles are assigned XCD-major (workgroup g runs on XCD g % 8), so
+// wavefront decodes 8 consecutive blocks and a 192-thread workgroup a tile
+// of 24. Tiles are assigned XCD-major (workgroup g runs on XCD g % 8), so
 // each XCD's L2 sees one contiguous range of the stream and cache lines
 // shared by neighbouring blocks are fetched from HBM once.
 //
@@ -13,10 +13,11 @@
 // decodes every block from LDS following unpackUint32Scalar
 // (scalar.go:260-309) and its validation order (unpack_block.go:127-237):
 //   validate -> unpack (one specialised path per bit width) -> exceptions
-//   (StreamVByte values decoded round-robin into LDS; bitmap patch by
-//   popcount rank, position-list patch by LDS OR) -> per-lane delta in
-//   registers (zigzag, or overflow checked, first wrap in v-major order)
-//   -> + FOR base -> 8-byte stores of the lane pairs.
+//   (the StreamVByte values of all eight blocks decoded by the whole wave
+//   from one flat table of control bytes; bitmap patch by popcount rank,
+//   position-list patch by LDS OR) -> per-lane delta in registers (zigzag,
+//   or overflow checked, first wrap in v-major order) -> + FOR base ->
+//   8-byte stores of the lane pairs.
 #include "utl_device.h"
 #include "utl_format.h"
 #include "utl_kernels.h"
@@ -31,12 +32,18 @@ constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 // block2Len (16) + payload (512) + bitmap (16) + control (32) + data (512).
 constexpr int kSubExtent = 1088;
 constexpr int kArena = 2560;        // staged bytes per wavefront (>= 15 + 2176 + 4)
+constexpr int kCtrlMax = kWaveBlocks * 32;  // StreamVByte control bytes of a wave's blocks
 
+// Exceptions of the wave's blocks are numbered flat: block b's control bytes
+// are entries [cbase_b, cbase_b + ctrl_b) of tab, with cbase_b the control
+// bytes of the blocks before it, and its exception i is slot 4*cbase_b + i of
+// aux. A block in position-list mode later reuses aux[128*b ..) to scatter.
 struct alignas(16) WaveLds {
   uint32_t arena[kArena / 4];
-  uint32_t aux[kWaveBlocks][kBlock];  // exception values by rank, or values (position list)
-  uint32_t tab[kWaveBlocks][32];      // per control byte: data offset | control byte << 24
+  uint32_t aux[kWaveBlocks * kBlock + 4];  // exception high bits by flat slot (+1 read past the last)
+  uint32_t tab[kCtrlMax];                  // per control byte: data offset | bw << 12 | control byte << 24
 };
+static_assert(sizeof(WaveLds) == 7696, "LDS per wavefront");
 
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -59,6 +66,16 @@ __device__ __forceinline__ uint32_t grp_incl_scan(uint32_t x, int j) {
 }
 
 __device__ __forceinline__ uint32_t grp_last(uint32_t x) { return (uint32_t)__shfl((int)x, lane_id() | 7, 64); }
+
+// Inclusive scan over the wave's eight 8-lane groups of a value that is equal
+// within each group (DPP row_shr:8, row_bcast:15, row_bcast:31). Must run
+// with all 64 lanes active.
+__device__ __forceinline__ uint32_t wave_group_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+  return x;
+}
 
 __device__ __forceinline__ uint32_t grp_min(uint32_t x) {
   x = min(x, (uint32_t)__shfl_xor((int)x, 1, 64));
@@ -140,32 +157,27 @@ struct SubResult {
   int consumed;
 };
 
-template <bool ForU64>
-__device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
-                                                   uint32_t* v1, SubResult& res);
+// Layout of one sub-block after the header checks of unpack_block.go:62-172.
+struct SubPlan {
+  Hdr h;
+  bool u64s;       // FOR64 single block (its base is applied by the caller)
+  int poff;        // payload offset from the sub-block's first byte
+  int svb_len;
+  uint32_t for_base;
+};
 
-// Decodes one staged sub-block whose first byte is at arena byte `at` and
-// whose true length is len. blk is the wave-local block index (LDS tables),
-// j the thread's index in the block. Returns the status word; count and
-// consumed are zero unless the status is UTL_OK.
+// Header checks, in the reference's order, up to the payload (per thread;
+// the early returns only leave this helper). Returns the status; *go is set
+// when the sub-block has values to decode (count > 0 and every check so far
+// passed). A count of 0 is a valid empty block of 4 bytes.
 template <bool ForU64>
-__device__ __forceinline__ int32_t decode_sub(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
-                                              uint32_t* v1, SubResult& res) {
-  SubResult r;
-  r.count = 0;
-  r.consumed = 0;
-  const int32_t st = decode_sub_impl<ForU64>(W, blk, at, len, j, v0, v1, r);
-  res.count = st == UTL_OK ? r.count : 0;
-  res.consumed = st == UTL_OK ? r.consumed : 0;
-  return st;
-}
-
-template <bool ForU64>
-__device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, int64_t len, int j, uint32_t* v0,
-                                                   uint32_t* v1, SubResult& res) {
-  const uint32_t* a = W->arena;
+__device__ __forceinline__ int32_t sub_prelude(const uint32_t* a, int at, int64_t len, SubPlan& p, bool& go,
+                                               int& consumed) {
+  go = false;
+  consumed = 0;
   if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
-  const Hdr h = decode_header(lds_u32(a, at));
+  p.h = decode_header(lds_u32(a, at));
+  const Hdr& h = p.h;
   bool combine = h.combine;
   if (ForU64) {
     if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
@@ -175,7 +187,7 @@ __device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, 
   }
   if ((unsigned)(h.count - 1) >= (unsigned)kBlock || (unsigned)h.bw > 32u) {
     if (h.count == 0) {
-      res.consumed = kHdr;
+      consumed = kHdr;
       return UTL_OK;
     }
     if (h.count > kBlock) return UTL_ERR_INVALID_BLOCK_LENGTH;
@@ -183,172 +195,268 @@ __device__ __forceinline__ int32_t decode_sub_impl(WaveLds* W, int blk, int at, 
   }
   const bool has_exc = h.exc > 0;
   const bool has_for = h.for_width > 0;
-  const bool u64s = ForU64 && is_for64_single(h.int_type, h.for_width, combine);
+  p.u64s = ForU64 && is_for64_single(h.int_type, h.for_width, combine);
   int poff = kHdr + (has_exc ? kSvbLen : 0);
   const int for_off = poff;
-  if (has_for) poff += u64s ? kFor64 : for_base_bytes(h.for_width);
+  if (has_for) poff += p.u64s ? kFor64 : for_base_bytes(h.for_width);
   if (combine) poff += kB2Len;
-  const int pbytes = h.bw * 16;
-  const int idx = has_exc ? exc_index_size(h.exc) : 0;
-  const int64_t end = (int64_t)poff + pbytes + idx;
+  p.poff = poff;
+  const int64_t end = (int64_t)poff + h.bw * 16 + (has_exc ? exc_index_size(h.exc) : 0);
   if (len < end) return UTL_ERR_INVALID_BUFFER;
-  int svb_len = 0;
+  p.svb_len = 0;
   if (has_exc) {
-    svb_len = (int)lds_bytes(a, at + kHdr, 2);
-    if (end + svb_len > len) return UTL_ERR_INVALID_BUFFER;
+    p.svb_len = (int)lds_bytes(a, at + kHdr, 2);
+    if (end + p.svb_len > len) return UTL_ERR_INVALID_BUFFER;
   }
-  const uint32_t for_base = (has_for && !u64s) ? lds_bytes(a, at + for_off, for_base_bytes(h.for_width)) : 0u;
+  p.for_base = (has_for && !p.u64s) ? lds_bytes(a, at + for_off, for_base_bytes(h.for_width)) : 0u;
+  go = true;
+  return UTL_OK;
+}
 
-  unpack_any(h.bw, a, at + poff, j, v0, v1);
+// Bitmap patch of one value pair: the pair's bits are the low two of `bits`;
+// x[rank] is the high part of the first set one (exceptions.go:246-284).
+__device__ __forceinline__ void patch_pair(const uint32_t* x, uint32_t rank, uint32_t bits, uint32_t& va,
+                                           uint32_t& vb) {
+  const uint32_t lo = x[rank], hi = x[rank + 1];
+  const bool s0 = (bits & 1u) != 0, s1 = (bits & 2u) != 0;
+  va |= s0 ? lo : 0u;
+  vb |= s1 ? (s0 ? hi : lo) : 0u;
+}
 
-  int consumed = poff + pbytes;
-  if (has_exc) {
-    // ---- exceptions (exceptions.go:32-88, :194-284) ----
-    if (h.exc > h.count) return UTL_ERR_INVALID_BUFFER;
-    const int ctrl = svb_ctrl_count(h.exc);
-    if (ctrl > svb_len) return UTL_ERR_INVALID_BUFFER;
-    const int exc_start = at + poff + pbytes;
-    const int svb_start = exc_start + idx;
-    // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15).
-    // Only the slots of exceptions < excCount count (exceptions.go:63-80):
-    // data bytes = valid slots + sum of their 2-bit codes, summed per byte
-    // with SWAR arithmetic.
-    const int nv = min(max(h.exc - 16 * j, 0), 16);  // valid slots in this dword
-    const uint32_t cw = nv > 0 ? lds_u32(a, svb_start + 4 * j) : 0u;
-    const uint32_t cm = cw & (nv >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nv)) - 1u));
-    const uint32_t t2 = (cm & 0x33333333u) + ((cm >> 2) & 0x33333333u);
-    const uint32_t t4 = (t2 & 0x0F0F0F0Fu) + ((t2 >> 4) & 0x0F0F0F0Fu);  // code sum per byte
-    const uint32_t pre = t4 * 0x01010100u;                                // exclusive byte prefix
-    const int part = nv + (int)((t4 * 0x01010101u) >> 24);
-    const uint32_t incl = grp_incl_scan((uint32_t)part, j);
-    const int total = (int)grp_last(incl);
-    if (ctrl + total > svb_len) return UTL_ERR_INVALID_BUFFER;
-    // Data offset of each owned control byte, with the byte itself.
-    const int off = svb_start + ctrl + (int)incl - part;
-    uint4 te;
-    te.x = (uint32_t)(off + ((pre >> 0) & 0xFF)) | ((cw & 0xFFu) << 24);
-    te.y = (uint32_t)(off + min(4, nv) + ((pre >> 8) & 0xFF)) | (((cw >> 8) & 0xFFu) << 24);
-    te.z = (uint32_t)(off + min(8, nv) + ((pre >> 16) & 0xFF)) | (((cw >> 16) & 0xFFu) << 24);
-    te.w = (uint32_t)(off + min(12, nv) + (pre >> 24)) | ((cw >> 24) << 24);
-    *reinterpret_cast<uint4*>(&W->tab[blk][4 * j]) = te;
+// Decodes one staged sub-block per 8-lane group whose first byte is at arena
+// byte `at` and whose true length is len; groups with act == false only take
+// part in the wave-wide steps. blk is the group's index in the wave, j the
+// thread's index in the group. Returns the status word; count and consumed
+// are zero unless the status is UTL_OK. Every lane of the wave must call it.
+template <bool ForU64>
+__device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int at, int64_t len, int j,
+                                              uint32_t* v0, uint32_t* v1, SubResult& res) {
+  const uint32_t* a = W->arena;
+  SubPlan p;
+  bool go = false;
+  int consumed = 0;
+  int32_t st = act ? sub_prelude<ForU64>(a, at, len, p, go, consumed) : UTL_OK;
+  res.count = 0;
+  res.consumed = 0;
+  if (go) unpack_any(p.h.bw, a, at + p.poff, j, v0, v1);
+
+  // ---- exception checks and this block's control-byte table (exceptions.go:32-88) ----
+  int ctrl_n = 0;  // control bytes this group contributes to the wave table
+  uint4 te = make_uint4(0, 0, 0, 0);
+  int exc_start = 0;
+  if (go) {
+    exc_start = at + p.poff + p.h.bw * 16;
+    consumed = p.poff + p.h.bw * 16;
+    if (p.h.exc > 0) {
+      const int exc = p.h.exc;
+      const int ctrl = svb_ctrl_count(exc);
+      if (exc > p.h.count || ctrl > p.svb_len) {
+        st = UTL_ERR_INVALID_BUFFER;
+      } else {
+        const int svb_start = exc_start + exc_index_size(exc);
+        // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15).
+        // Only the slots of exceptions < excCount count (exceptions.go:63-80):
+        // data bytes = valid slots + sum of their 2-bit codes, summed per
+        // byte with SWAR arithmetic.
+        const int nv = min(max(exc - 16 * j, 0), 16);  // valid slots in this dword
+        const uint32_t cw = nv > 0 ? lds_u32(a, svb_start + 4 * j) : 0u;
+        const uint32_t cm = cw & (nv >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nv)) - 1u));
+        const uint32_t t2 = (cm & 0x33333333u) + ((cm >> 2) & 0x33333333u);
+        const uint32_t t4 = (t2 & 0x0F0F0F0Fu) + ((t2 >> 4) & 0x0F0F0F0Fu);  // code sum per byte
+        const uint32_t pre = t4 * 0x01010100u;                                // exclusive byte prefix
+        const int part = nv + (int)((t4 * 0x01010101u) >> 24);
+        const uint32_t incl = grp_incl_scan((uint32_t)part, j);
+        const int total = (int)grp_last(incl);
+        if (ctrl + total > p.svb_len) {
+          st = UTL_ERR_INVALID_BUFFER;
+        } else {
+          // Data offset of each owned control byte, with its bw and the byte.
+          const uint32_t off = (uint32_t)(svb_start + ctrl + (int)incl - part);
+          const uint32_t bwf = (uint32_t)p.h.bw << 12;
+          te.x = (off + ((pre >> 0) & 0xFF)) | bwf | ((cw & 0xFFu) << 24);
+          te.y = (off + min(4, nv) + ((pre >> 8) & 0xFF)) | bwf | (((cw >> 8) & 0xFFu) << 24);
+          te.z = (off + min(8, nv) + ((pre >> 16) & 0xFF)) | bwf | (((cw >> 16) & 0xFFu) << 24);
+          te.w = (off + min(12, nv) + (pre >> 24)) | bwf | ((cw >> 24) << 24);
+          ctrl_n = ctrl;
+          consumed = svb_start + p.svb_len - at;
+        }
+      }
+    }
+  }
+
+  // ---- wave-wide StreamVByte decode of every group's exceptions ----
+  const uint32_t cincl = wave_group_scan((uint32_t)ctrl_n);
+  const int cbase = (int)cincl - ctrl_n;
+  const int ctot = __builtin_amdgcn_readlane((int)cincl, 63);
+  if (ctot > 0) {
+    if (ctrl_n > 0) {
+      const int c0 = 4 * j;
+      uint32_t* t = &W->tab[cbase + c0];
+      if (c0 < ctrl_n) t[0] = te.x;
+      if (c0 + 1 < ctrl_n) t[1] = te.y;
+      if (c0 + 2 < ctrl_n) t[2] = te.z;
+      if (c0 + 3 < ctrl_n) t[3] = te.w;
+    }
     wave_fence();
-    // StreamVByte values, round-robin: exception i = 8m + j.
-    const int slot = j & 3;
+    // Slot f belongs to control byte f >> 2, code slot f & 3 (constant per
+    // lane since f advances by 64). Slots past a block's excCount decode
+    // bytes nobody reads.
+    const int slot = lane_id() & 3;
     const uint32_t pre_mask = (1u << (2 * slot)) - 1u;
+    const int nslots = 4 * ctot;
 #pragma unroll 1
-    for (int i = j; i < h.exc; i += kTpb) {
-      const uint32_t e = W->tab[blk][i >> 2];
+    for (int f = lane_id(); f < nslots; f += 64) {
+      const uint32_t e = W->tab[f >> 2];
       const uint32_t c = e >> 24;
       const uint32_t low = c & pre_mask;
       const int pre = slot + (int)(low & 3) + (int)((low >> 2) & 3) + (int)((low >> 4) & 3);
       const int l = (int)((c >> (2 * slot)) & 3) + 1;
-      W->aux[blk][i] = go_shl32(lds_bytes(a, (int)(e & 0xFFFFFF) + pre, l), h.bw);
+      W->aux[f] = go_shl32(lds_bytes(a, (int)(e & 0xFFF) + pre, l), (int)((e >> 12) & 63));
     }
     wave_fence();
-    if (h.exc <= kExcBitmap) {
-      // Sorted position list: OR each exception into its position in LDS.
-      // Full blocks mask positions with 0x7F, partial blocks skip positions
-      // >= count (exceptions.go:211-230).
-      const uint32_t hv0 = j < h.exc ? W->aux[blk][j] : 0u;
-      const uint32_t hv1 = j + 8 < h.exc ? W->aux[blk][j + 8] : 0u;
-      wave_fence();
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        W->aux[blk][16 * r + 2 * j] = v0[r];
-        W->aux[blk][16 * r + 2 * j + 1] = v1[r];
-      }
-      wave_fence();
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int i = j + 8 * e;
-        if (i < h.exc) {
-          const uint32_t p = lds_bytes(a, exc_start + i, 1);
-          const uint32_t hv = e ? hv1 : hv0;
-          if (h.count == kBlock) atomicOr(&W->aux[blk][p & 0x7F], hv);
-          else if ((int)p < h.count) atomicOr(&W->aux[blk][p], hv);
-        }
-      }
-      wave_fence();
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        v0[r] = W->aux[blk][16 * r + 2 * j];
-        v1[r] = W->aux[blk][16 * r + 2 * j + 1];
-      }
+  }
+
+  // ---- patch (exceptions.go:194-284) ----
+  const bool patch = ctrl_n > 0;
+  const bool list = patch && p.h.exc <= kExcBitmap;
+  uint32_t hv0 = 0, hv1 = 0;
+  if (patch) {
+    const uint32_t* x = &W->aux[4 * cbase];
+    if (list) {
+      hv0 = j < p.h.exc ? x[j] : 0u;
+      hv1 = j + 8 < p.h.exc ? x[j + 8] : 0u;
     } else {
       // 128-bit bitmap masked to count; the rank of a set bit is the index
       // of its high bits. popcount(bitmap) > excCount applies nothing.
       uint32_t bm[4];
 #pragma unroll
       for (int d = 0; d < 4; ++d) bm[d] = lds_u32(a, exc_start + 4 * d);
-      if (h.count < kBlock) {
+      if (p.h.count < kBlock) {
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-          const int keep = h.count - 32 * d;
+          const int keep = p.h.count - 32 * d;
           bm[d] = keep >= 32 ? bm[d] : (keep <= 0 ? 0u : (bm[d] & ((1u << keep) - 1u)));
         }
       }
       const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
-      if ((int)(pre3 + __popc(bm[3])) <= h.exc) {
-        const int b0 = 2 * j;
+      if ((int)(pre3 + __popc(bm[3])) <= p.h.exc) {
+        // word d holds rows 2d (bits 2j, 2j+1) and 2d+1 (bits 16+2j, 17+2j)
+        const int ba = 2 * j, bb = 16 + 2 * j;
+        const uint32_t ma = (1u << ba) - 1u, mb = (1u << bb) - 1u;
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-          const int d = r >> 1;
-          const uint32_t word = bm[d];
-          const uint32_t pre = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-          const int bit = b0 + 16 * (r & 1);
-          const uint32_t rank0 = pre + __popc(word & ((1u << bit) - 1u));
-          const uint32_t s0 = (word >> bit) & 1u, s1 = (word >> (bit + 1)) & 1u;
-          const uint32_t x0 = W->aux[blk][rank0 & 127], x1 = W->aux[blk][(rank0 + s0) & 127];
-          v0[r] |= s0 ? x0 : 0u;
-          v1[r] |= s1 ? x1 : 0u;
+        for (int d = 0; d < 4; ++d) {
+          const uint32_t w = bm[d];
+          const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
+          patch_pair(x, pd + __popc(w & ma), w >> ba, v0[2 * d], v1[2 * d]);
+          patch_pair(x, pd + __popc(w & mb), w >> bb, v0[2 * d + 1], v1[2 * d + 1]);
         }
       }
     }
-    consumed = svb_start + svb_len - at;
+  }
+  // Sorted position list: OR each exception into its position in LDS. Full
+  // blocks mask positions with 0x7F, partial blocks skip positions >= count
+  // (exceptions.go:211-230). Every bitmap read of aux is done before the
+  // scatter reuses aux[128*blk ..).
+  if (__ballot(list)) {
+    wave_fence();
+    uint32_t* y = &W->aux[kBlock * blk];
+    if (list) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) *reinterpret_cast<uint2*>(&y[16 * r + 2 * j]) = make_uint2(v0[r], v1[r]);
+    }
+    wave_fence();
+    if (list) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int i = j + 8 * e;
+        if (i < p.h.exc) {
+          const uint32_t q = lds_bytes(a, exc_start + i, 1);
+          const uint32_t hv = e ? hv1 : hv0;
+          if (p.h.count == kBlock) atomicOr(&y[q & 0x7F], hv);
+          else if ((int)q < p.h.count) atomicOr(&y[q], hv);
+        }
+      }
+    }
+    wave_fence();
+    if (list) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const uint2 t = *reinterpret_cast<const uint2*>(&y[16 * r + 2 * j]);
+        v0[r] = t.x;
+        v1[r] = t.y;
+      }
+    }
   }
 
-  // ---- per-lane delta decode (delta.go:50-115) ----
-  if (h.delta) {
-    if (h.zigzag) {
-      v0[0] = zz_dec(v0[0]);
-      v1[0] = zz_dec(v1[0]);
+  if (!go) {
+    res.consumed = st == UTL_OK ? consumed : 0;  // count 0: a valid 4-byte block
+    return st;
+  }
+  if (st != UTL_OK) return st;
+
+  // ---- per-lane delta decode (delta.go:50-115) and FOR (for.go:136-140) ----
+  const uint32_t fb = (p.h.for_width > 0 && !p.u64s) ? p.for_base : 0u;
+  if (p.h.delta) {
+    if (p.h.zigzag) {
+      v0[0] = zz_dec(v0[0]) + fb;
+      v1[0] = zz_dec(v1[0]) + fb;
 #pragma unroll
       for (int r = 1; r < kRows; ++r) {
         v0[r] = v0[r - 1] + zz_dec(v0[r]);
         v1[r] = v1[r - 1] + zz_dec(v1[r]);
       }
     } else {
-      uint64_t any = 0;  // lanes of the wave with a wrapped addition
+      // A lane's running sum can wrap only if its 8 addends reach 2^32, which
+      // needs one of them >= 2^29. Without such a value anywhere in the wave
+      // the prefix runs unchecked, with the base folded into row 0.
+      uint32_t big = 0;
 #pragma unroll
-      for (int r = 1; r < kRows; ++r) {
-        const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
-        any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
-        v0[r] = n0;
-        v1[r] = n1;
-      }
-      if (any) {
-        // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
-        // first wrap at a position < count in v-major order wins.
-        uint32_t first = 0xFFFFFFFFu;
+      for (int r = 0; r < kRows; ++r) big |= v0[r] | v1[r];
+      if (__ballot(big >= (1u << 29)) == 0) {
+        v0[0] += fb;
+        v1[0] += fb;
 #pragma unroll
-        for (int r = kRows - 1; r >= 1; --r) {
-          const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
-          if (v1[r] < v1[r - 1] && (int)p1 < h.count) first = p1;
-          if (v0[r] < v0[r - 1] && (int)p0 < h.count) first = p0;
+        for (int r = 1; r < kRows; ++r) {
+          v0[r] += v0[r - 1];
+          v1[r] += v1[r - 1];
         }
-        const uint32_t m = grp_min(first);
-        if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+      } else {
+        uint64_t any = 0;  // lanes of the wave with a wrapped addition
+#pragma unroll
+        for (int r = 1; r < kRows; ++r) {
+          const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
+          any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
+          v0[r] = n0;
+          v1[r] = n1;
+        }
+        if (any) {
+          // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
+          // first wrap at a position < count in v-major order wins.
+          uint32_t first = 0xFFFFFFFFu;
+#pragma unroll
+          for (int r = kRows - 1; r >= 1; --r) {
+            const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
+            if (v1[r] < v1[r - 1] && (int)p1 < p.h.count) first = p1;
+            if (v0[r] < v0[r - 1] && (int)p0 < p.h.count) first = p0;
+          }
+          const uint32_t m = grp_min(first);
+          if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+        }
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          v0[r] += fb;
+          v1[r] += fb;
+        }
       }
     }
-  }
-  if (has_for && !u64s) {
+  } else if (fb != 0) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      v0[r] += for_base;
-      v1[r] += for_base;
+      v0[r] += fb;
+      v1[r] += fb;
     }
   }
-  res.count = h.count;
+  res.count = p.h.count;
   res.consumed = consumed;
   return UTL_OK;
 }
@@ -367,22 +475,40 @@ __device__ __forceinline__ int64_t clamp_extent(int64_t len) {
 // aligned chunk that holds one of the bytes, spread over the wave.
 // All of a thread's chunk loads are issued before the first LDS write, so a
 // staged range costs one round trip to HBM.
-__device__ __forceinline__ void stage_range(WaveLds* W, const uint8_t* __restrict__ g, int64_t n) {
+// The chunks of one staged range, loaded into registers (scalar members: a
+// uint4 array here is placed in scratch).
+struct Stage {
+  uint4 c0, c1, c2;
+  int nchunks;
+};
+
+__device__ __forceinline__ Stage stage_issue(const uint8_t* __restrict__ g, int64_t n) {
   static_assert((kArena / 16 + 63) / 64 == 3, "three chunks per thread");
   const int s = (int)(reinterpret_cast<uintptr_t>(g) & 15);
   const uint4* base = reinterpret_cast<const uint4*>(g - s);
-  const int nchunks = n > 0 ? (int)((s + n + 15) >> 4) : 0;
+  Stage st;
+  st.nchunks = n > 0 ? (int)((s + n + 15) >> 4) : 0;
   const int l = lane_id();
-  // (scalar variables: a uint4 array here is placed in scratch)
-  uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, c2 = c0;
-  if (l < nchunks) c0 = base[l];
-  if (l + 64 < nchunks) c1 = base[l + 64];
-  if (l + 128 < nchunks) c2 = base[l + 128];
+  st.c0 = make_uint4(0, 0, 0, 0);
+  st.c1 = st.c0;
+  st.c2 = st.c0;
+  if (l < st.nchunks) st.c0 = base[l];
+  if (l + 64 < st.nchunks) st.c1 = base[l + 64];
+  if (l + 128 < st.nchunks) st.c2 = base[l + 128];
+  return st;
+}
+
+__device__ __forceinline__ void stage_commit(WaveLds* W, const Stage& st) {
+  const int l = lane_id();
   uint4* ar = reinterpret_cast<uint4*>(W->arena);
-  if (l < nchunks) ar[l] = c0;
-  if (l + 64 < nchunks) ar[l + 64] = c1;
-  if (l + 128 < nchunks) ar[l + 128] = c2;
+  if (l < st.nchunks) ar[l] = st.c0;
+  if (l + 64 < st.nchunks) ar[l + 64] = st.c1;
+  if (l + 128 < st.nchunks) ar[l + 128] = st.c2;
   wave_fence();
+}
+
+__device__ __forceinline__ void stage_range(WaveLds* W, const uint8_t* __restrict__ g, int64_t n) {
+  stage_commit(W, stage_issue(g, n));
 }
 
 __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t out0, int count, int j,
@@ -429,54 +555,78 @@ struct U64Pending {
   bool pending;   // sub-block 2 lies outside the staged extent
 };
 
-// uint64.go:191-266 unpackUint64Block, first part: everything up to the
-// second sub-block of a two-block block. Returns the status; on a two-block
-// block whose sub-block 2 is staged it is decoded here too.
-__device__ __forceinline__ int32_t decode_u64_block(WaveLds* W, int blk, int at, int64_t len, int64_t avail, int j, uint32_t* l0,
-                                    uint32_t* l1, uint32_t* h0, uint32_t* h1, uint64_t& base, bool& two, int& count,
-                                    int& consumed, U64Pending& pend) {
+// uint64.go:191-266 unpackUint64Block up to the second sub-block of a
+// two-block block, which is decoded here too when it is staged. Returns the
+// status. Every lane of the wave must call it (act selects the groups that
+// decode).
+__device__ __forceinline__ int32_t decode_u64_block(WaveLds* W, bool act, int blk, int at, int64_t len,
+                                                    int64_t avail, int j, uint32_t* l0, uint32_t* l1, uint32_t* h0,
+                                                    uint32_t* h1, uint64_t& base, bool& two, int& count,
+                                                    int& consumed, U64Pending& pend) {
   two = false;
   base = 0;
   count = 0;
   consumed = 0;
   pend.pending = false;
-  if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
-  const Hdr h = decode_header(lds_u32(W->arena, at));
-  if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
-  SubResult r1;
-  if (!h.combine) {
-    if (is_for64_single(h.int_type, h.for_width, h.combine)) {
-      const int bo = kHdr + (h.exc > 0 ? kSvbLen : 0);
-      if (len < bo + kFor64) return UTL_ERR_INVALID_BUFFER;
-      base = (uint64_t)lds_u32(W->arena, at + bo) | ((uint64_t)lds_u32(W->arena, at + bo + 4) << 32);
+  int32_t st = UTL_OK;
+  bool go1 = act;
+  Hdr h = decode_header(0);
+  if (go1) {
+    if (len < kHdr) {
+      st = UTL_ERR_INVALID_BUFFER;
+    } else {
+      h = decode_header(lds_u32(W->arena, at));
+      if (h.int_type != 3) {
+        st = UTL_ERR_UNSUPPORTED_TYPE;
+      } else if (!h.combine && is_for64_single(h.int_type, h.for_width, h.combine)) {
+        const int bo = kHdr + (h.exc > 0 ? kSvbLen : 0);
+        if (len < bo + kFor64)
+          st = UTL_ERR_INVALID_BUFFER;
+        else
+          base = (uint64_t)lds_u32(W->arena, at + bo) | ((uint64_t)lds_u32(W->arena, at + bo + 4) << 32);
+      }
     }
-    const int32_t st = decode_sub<true>(W, blk, at, len, j, l0, l1, r1);
-    count = r1.count;
-    consumed = r1.consumed;
-    return st;
+    go1 = st == UTL_OK;
   }
-  int32_t st = decode_sub<true>(W, blk, at, len, j, l0, l1, r1);
-  if (st) return st;
-  const bool he = h.exc > 0;
-  const int forb = for_base_bytes(h.for_width);
-  if (len < payload_offset(forb, he, true)) return UTL_ERR_INVALID_BUFFER;
-  const int b2len = (int)lds_bytes(W->arena, at + kHdr + (he ? kSvbLen : 0) + forb, 2);
-  const int64_t b2s = r1.consumed;
-  if (b2s + b2len > len) return UTL_ERR_INVALID_BUFFER;
-  two = true;
-  count = r1.count;
-  pend.c1 = r1.consumed;
-  pend.b2len = b2len;
-  if (b2s + clamp_extent<false>(b2len) > avail) {
-    pend.pending = true;
-    return UTL_OK;
+  SubResult r1;
+  const int32_t s1 = decode_sub<true>(W, go1, blk, at, len, j, l0, l1, r1);
+  bool go2 = false;
+  int b2len = 0;
+  if (go1) {
+    st = s1;
+    if (st == UTL_OK && !h.combine) {
+      count = r1.count;
+      consumed = r1.consumed;
+    } else if (st == UTL_OK) {
+      const bool he = h.exc > 0;
+      const int forb = for_base_bytes(h.for_width);
+      if (len < payload_offset(forb, he, true)) {
+        st = UTL_ERR_INVALID_BUFFER;
+      } else {
+        b2len = (int)lds_bytes(W->arena, at + kHdr + (he ? kSvbLen : 0) + forb, 2);
+        if ((int64_t)r1.consumed + b2len > len) {
+          st = UTL_ERR_INVALID_BUFFER;
+        } else {
+          two = true;
+          count = r1.count;
+          pend.c1 = r1.consumed;
+          pend.b2len = b2len;
+          if ((int64_t)r1.consumed + clamp_extent<false>(b2len) > avail)
+            pend.pending = true;
+          else
+            go2 = true;
+        }
+      }
+    }
   }
   SubResult r2;
-  st = decode_sub<true>(W, blk, at + (int)b2s, b2len, j, h0, h1, r2);
-  if (st) return st;
-  if (r2.count != r1.count) return UTL_ERR_INVALID_BUFFER;
-  consumed = r1.consumed + b2len;
-  return UTL_OK;
+  const int32_t s2 = decode_sub<true>(W, go2, blk, at + (go2 ? r1.consumed : 0), b2len, j, h0, h1, r2);
+  if (go2) {
+    st = s2;
+    if (st == UTL_OK && r2.count != r1.count) st = UTL_ERR_INVALID_BUFFER;
+    if (st == UTL_OK) consumed = r1.consumed + b2len;
+  }
+  return st;
 }
 
 // Decodes the wave's blocks [b0, b0 + nb). The wave-uniform round logic
@@ -541,10 +691,10 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
     int32_t st = UTL_OK;
     int consumed = 0;
     if (!U64) {
+      uint32_t v0[kRows], v1[kRows];
+      SubResult r1;
+      st = decode_sub<false>(W, active, bb, active ? at : 0, len, jj, v0, v1, r1);
       if (active) {
-        uint32_t v0[kRows], v1[kRows];
-        SubResult r1;
-        st = decode_sub<false>(W, bb, at, len, jj, v0, v1, r1);
         consumed = r1.consumed;
         if (st == UTL_OK && (uint64_t)r1.count > slot_n) st = UTL_ERR_ARGUMENT;
         if (st == UTL_OK && r1.count > 0) store_u32((uint32_t*)out, out0, r1.count, jj, v0, v1);
@@ -560,10 +710,8 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
       bool two = false;
       int count = 0;
       U64Pending pend;
-      pend.pending = false;
-      if (active)
-        st = decode_u64_block(W, bb, at, len, clamp_extent<true>(len), jj, l0, l1, h0, h1, base64, two, count,
-                              consumed, pend);
+      st = decode_u64_block(W, active, bb, active ? at : 0, len, clamp_extent<true>(len), jj, l0, l1, h0, h1,
+                            base64, two, count, consumed, pend);
       wave_fence();
       // Corrupt blocks only: sub-block 2 outside the staged extent gets a
       // staged round of its own, one block at a time.
@@ -575,9 +723,12 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
         const int c1 = __shfl(pend.c1, k, 64), b2len = __shfl(pend.b2len, k, 64);
         const uint8_t* g = src + boff[kb] + c1;
         stage_range(W, g, clamp_extent<false>(b2len));
-        if (bb == (k >> 3)) {
-          SubResult r2;
-          st = decode_sub<true>(W, bb, (int)(reinterpret_cast<uintptr_t>(g) & 15), b2len, jj, h0, h1, r2);
+        const bool me = bb == (k >> 3);
+        SubResult r2;
+        const int32_t s2 =
+            decode_sub<true>(W, me, bb, (int)(reinterpret_cast<uintptr_t>(g) & 15), b2len, jj, h0, h1, r2);
+        if (me) {
+          st = s2;
           if (st == UTL_OK && r2.count != count) st = UTL_ERR_INVALID_BUFFER;
           if (st == UTL_OK) consumed = c1 + b2len;
         }
@@ -602,7 +753,7 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
 __device__ __forceinline__ uint64_t xcd_tile(uint64_t g, uint64_t per_xcd) { return (g & 7) * per_xcd + (g >> 3); }
 
 template <bool U64>
-__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
+__global__ __launch_bounds__(64 * kWgWaves) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                                 const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                 void* __restrict__ out, int32_t* __restrict__ status,
                                                 int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
@@ -619,7 +770,7 @@ __global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ src,
 // own work counter (counter[8x]) over XCD x's contiguous range of tiles, so
 // load balance stays inside the XCD. Every workgroup reaches the exit test.
 template <bool U64>
-__global__ __launch_bounds__(256) void k_decode_persistent(const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(64 * kWgWaves) void k_decode_persistent(const uint8_t* __restrict__ src,
                                                            const uint64_t* __restrict__ boff,
                                                            const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                            void* __restrict__ out, int32_t* __restrict__ status,
@@ -681,11 +832,12 @@ hipError_t utl_launch_decode_persistent(bool u64, const uint8_t* src, const uint
   const uint64_t per_xcd = (ntiles + 7) / 8;
   const unsigned grid = (unsigned)((tune.grid_wgs + 7) / 8 * 8);
   const uint32_t grab = tune.blocks_per_wave ? tune.blocks_per_wave : 1;
+  const dim3 block(64 * kWgWaves);
   if (u64)
-    k_decode_persistent<true><<<dim3(grid), dim3(256), 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
-                                                                     counter, per_xcd, grab);
+    k_decode_persistent<true><<<dim3(grid), block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
+                                                                 counter, per_xcd, grab);
   else
-    k_decode_persistent<false><<<dim3(grid), dim3(256), 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
-                                                                      counter, per_xcd, grab);
+    k_decode_persistent<false><<<dim3(grid), block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
+                                                                  counter, per_xcd, grab);
   return hipGetLastError();
 }
