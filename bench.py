@@ -80,8 +80,11 @@ def main() -> int:
     ap.add_argument("--values", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step (for profiling)")
-    ap.add_argument("--workload", default="delta64m", choices=["delta64m", "plain1m", "uneven", "u64"],
+    ap.add_argument("--workload", default="delta64m", choices=["delta64m", "plain1m", "uneven", "u64", "serve"],
                     help="delta64m is the headline; the others print their own JSON lines")
+    ap.add_argument("--store-gb", type=float, default=16.0, help="serve: compressed store size")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: offered requests per second")
+    ap.add_argument("--requests", type=int, default=4000, help="serve: requests per run")
     ap.add_argument("--streams", type=int, default=200_000, help="streams of the uneven workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB copy kernel (HBM counter calibration)")
@@ -105,6 +108,9 @@ def main() -> int:
         elif args.workload == "u64":
             for rec in runbench.u64_and_get(ctx):
                 print(json.dumps(rec), flush=True)
+        elif args.workload == "serve":
+            print(json.dumps(runbench.serving(ctx, store_bytes=int(args.store_gb * (1 << 30)), rate=args.rate,
+                                              nreq=args.requests, progress=log)), flush=True)
         return 0
     n = args.values
     log(f"generating {n} delta postings")

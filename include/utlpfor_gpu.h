@@ -175,6 +175,20 @@ int utl_get_batch(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t
                   const uint64_t *d_out_off, size_t nblocks, const uint64_t *d_req_block, const int64_t *d_req_pos,
                   size_t nreq, void *d_out, int32_t *d_status);
 
+/* Gathered index for a serving batch (no reference counterpart): request r,
+ * row {first block, block count n, value base u, entry base e} of d_req
+ * (nreq rows of 4 uint64), fills entries [e, e + n] of d_gblock_off /
+ * d_gout_off from the store index d_block_off / d_out_off (store_nblocks
+ * blocks), with out offsets rebased to u. Rows must satisfy
+ * e_0 = 0, e_{r+1} = e_r + n_r + 1, u_{r+1} = u_r + (values of r), and
+ * nentries = e_last + n_last + 1. Decoding the gathered index over
+ * nentries - 1 blocks (utl_decode_u32/u64) then decodes every request into
+ * out[u_r ..]; the block between two requests has an empty output slot and
+ * writes nothing, and its status belongs to no request. */
+int utl_gather_index(utl_ctx *ctx, const uint64_t *d_block_off, const uint64_t *d_out_off, size_t store_nblocks,
+                     const uint64_t *d_req, size_t nreq, size_t nentries, uint64_t *d_gblock_off,
+                     uint64_t *d_gout_off);
+
 /* Batched PackUint32 / PackUint64 (pack.go:76, uint64.go:279) over a job
  * table: block b encodes values[d_in_off[b]..] with
  * d_in_meta[b] = count (1..128) | flag << 8 into slot b of d_slots

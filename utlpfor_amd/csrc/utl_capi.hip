@@ -493,6 +493,19 @@ int utl_get_tune(utl_ctx* ctx, int* waves_per_wg, int* grid_wgs, int* blocks_per
   return UTL_OK;
 }
 
+int utl_gather_index(utl_ctx* ctx, const uint64_t* d_block_off, const uint64_t* d_out_off, size_t store_nblocks,
+                     const uint64_t* d_req, size_t nreq, size_t nentries, uint64_t* d_gblock_off,
+                     uint64_t* d_gout_off) {
+  if (nentries && (!d_block_off || !d_out_off || !d_req || !nreq || !d_gblock_off || !d_gout_off))
+    return UTL_ERR_ARGUMENT;
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_gather_index(d_block_off, d_out_off, store_nblocks, d_req, nreq, nentries, d_gblock_off,
+                                  d_gout_off, c->stream));
+  return UTL_OK;
+}
+
 int utl_get_batch(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const uint64_t* d_block_off,
                   const uint64_t* d_out_off, size_t nblocks, const uint64_t* d_req_block, const int64_t* d_req_pos,
                   size_t nreq, void* d_out, int32_t* d_status) {

@@ -32,3 +32,8 @@ hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t 
 
 // Plain streaming copy (16-byte vectors when both pointers allow it).
 hipError_t utl_launch_copy(uint8_t* dst, const uint8_t* src, uint64_t n, hipStream_t stream);
+
+// Gathered index of a batch of block ranges of a resident store (utl_serve.hip).
+hipError_t utl_launch_gather_index(const uint64_t* boff, const uint64_t* ooff, uint64_t store_nblocks,
+                                   const uint64_t* req, uint64_t nreq, uint64_t nentries, uint64_t* gboff,
+                                   uint64_t* gooff, hipStream_t stream);

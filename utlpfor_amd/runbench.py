@@ -74,6 +74,66 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3) 
     return rec
 
 
+def serving(ctx: Context, store_bytes: int = 16 << 30, rate: float = 2000.0, nreq: int = 4000, seed: int = 5,
+            check_every: int = 25, progress=None) -> dict:
+    """Continuous-batching serving over a resident store of `store_bytes`
+    compressed bytes (generators plain / delta postings / timestamps64 /
+    uniform64 encoded once, replicated to fill): Poisson arrivals at `rate`
+    per second, 60% decompress / 25% compress / 15% Get (1-1024 positions),
+    sizes log-uniform in [128, 4Mi] values. Served twice with the same
+    requests: results copied to host memory, and results left in HBM.
+    Every `check_every`-th request is checked against the source values (or
+    the oracle's packed bytes for compress)."""
+    from . import serve
+    t0 = time.time()
+    sources = [workloads.plain_1m(8 << 20, seed=seed), workloads.delta_postings(32 << 20, seed=seed + 1),
+               workloads.timestamps64(16 << 20, seed=seed + 2), workloads.uniform64(4 << 20, seed=seed + 3)]
+    flags = [0, 1, 0, 0]
+    store = serve.Store(ctx, sources, flags, target_bytes=store_bytes)
+    if progress:
+        progress(f"store: {len(store.streams)} streams, {store.nblocks} blocks, {store.nbytes / 1e9:.2f} GB "
+                 f"in {time.time() - t0:.1f} s")
+    pool = [(workloads.plain_1m(4 << 20, seed=seed + 10), 0), (workloads.delta_postings(8 << 20, seed=seed + 11), 1),
+            (workloads.timestamps64(4 << 20, seed=seed + 12), 0), (workloads.uniform64(1 << 20, seed=seed + 13), 0)]
+    out = {"workload": "serving: Poisson arrivals, 60% decompress / 25% compress / 15% Get(1-1024), "
+                       "sizes logU[128, 4Mi] values, mixed u32/u64",
+           "store_bytes": store.nbytes, "streams": len(store.streams), "offered_rate_per_s": rate}
+    for deliver in (True, False):
+        reqs = serve.make_requests(store, pool, n=nreq, rate=rate, seed=seed + 20)
+        eng = serve.Engine(ctx, store, heap_fraction=0.25)
+        eng.run_batch(reqs[:8], deliver)  # warm-up tick
+        stats = eng.serve(reqs, deliver=deliver)
+        stats["heap_bytes"] = eng.heap.cap
+        if deliver:
+            stats["bit_exact_sampled"] = _check_served(store, reqs[::check_every])
+        out["to_host" if deliver else "in_hbm"] = stats
+        eng.heap.mem.free()
+        if progress:
+            progress(f"served ({'to host' if deliver else 'in HBM'}): {stats}")
+    store.free()
+    return out
+
+
+def _check_served(store, reqs) -> bool:
+    from oracle import oracle
+    from . import serve
+    ok = True
+    for r in reqs:
+        if r.kind == serve.DECOMPRESS:
+            st = store.streams[r.stream]
+            want = store.expected(r.stream, r.first * 128, r.first * 128 + r.result.size)
+            good = np.repeat(r.status == 0, 128)[:r.result.size]
+            ok &= bool(np.array_equal(r.result[good], want[good]))
+            ok &= bool(st.is64 or np.all(r.status == 0))
+        elif r.kind == serve.GET:
+            src = store.sources[store.streams[r.stream].source]
+            good = r.status == 0
+            ok &= bool(np.array_equal(r.result[good], src[r.positions][good]))
+        else:
+            ok &= bool(np.array_equal(r.result, oracle.pack_stream(r.flag, r.values)[0]))
+    return ok
+
+
 def u64_and_get(ctx: Context, reps: int = 10, n_ts: int = 32 << 20, n_uni: int = 8 << 20,
                 n_get: int = 10_000_000) -> list:
     recs = []

@@ -1,0 +1,396 @@
+"""Serving loop over a compressed store resident in HBM (continuous batching).
+
+The reference is a library of per-block calls; a server built on it answers
+each request with a loop of UnpackUint32 / PackUint32 / GetUint32 calls
+(unpack.go:5, pack.go:76, get.go:74). Here requests are batched instead:
+
+* Store: many streams (uint32 and uint64, each a concatenation of blocks
+  exactly as PackUint32(flag|Append) builds it) laid end to end in one HBM
+  buffer, with one global index (block byte offsets and value offsets from
+  the device scan of the encoders' per-block meta words).
+* Requests: decompress a block range of a stream, compress a value array,
+  or Get up to 1024 positions of a stream.
+* Engine: each tick admits the requests that have arrived, in arrival
+  order, while their device memory fits the engine's heap (a fixed share of
+  free HBM taken at start-up; admission by free HBM), then runs the whole
+  batch with one launch per kind and type: a gathered index
+  (utl_gather_index) plus one decode for all decompress requests, one
+  batched Get, one batched encode. Requests that arrive meanwhile join the
+  next tick. A request's latency is from its arrival to the end of the tick
+  that answers it, including the copy of its result to the host.
+
+Every result equals what the request gets when it runs alone (the kernels
+treat blocks independently); tests check that bit for bit.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, batch
+from .device import Context, DeviceArray, check, mem_info
+
+DECOMPRESS, COMPRESS, GET = 0, 1, 2
+KIND_NAMES = ("decompress", "compress", "get")
+_ALIGN = 256
+
+
+@dataclass
+class StoreStream:
+    first_block: int   # global block index of the stream's first block
+    nblocks: int
+    nvalues: int
+    is64: bool
+    source: int        # index of the source array the stream encodes
+    flag: int
+
+
+class Store:
+    """Streams resident in HBM with one global index. Each source array is
+    encoded once on the GPU; the store holds `replicas` copies of every
+    encoded source (device-to-device copies), or as many as fill
+    `target_bytes` of compressed data."""
+
+    def __init__(self, ctx: Context, sources: list[np.ndarray], flags: list[int], replicas: int = 1,
+                 target_bytes: int | None = None):
+        lib = _lib.load()
+        self.ctx = ctx
+        self.sources = sources
+        enc = [batch.encode(ctx, s, flag=f) for s, f in zip(sources, flags)]
+        per_round = sum(e.nbytes for e in enc)
+        if target_bytes is not None:
+            replicas = max(1, int(target_bytes // max(per_round, 1)))
+        self.replicas = replicas
+        nbytes = per_round * replicas
+        nblocks = sum(e.nblocks for e in enc) * replicas
+        self.data = DeviceArray(nbytes + 64, np.uint8, ctx)
+        self.meta = DeviceArray(max(nblocks, 1), np.uint32, ctx)
+        self.streams: list[StoreStream] = []
+        boff = blk = 0
+        for _ in range(replicas):
+            for i, e in enumerate(enc):
+                check(lib.utl_dev_copy(ctx.handle, self.data.ptr + boff, e.data.ptr, e.nbytes), "store copy")
+                check(lib.utl_dev_copy(ctx.handle, self.meta.ptr + 4 * blk, e.meta.ptr, 4 * e.nblocks), "meta copy")
+                self.streams.append(StoreStream(blk, e.nblocks, e.nvalues, e.is64, i, flags[i]))
+                boff += e.nbytes
+                blk += e.nblocks
+        self.nbytes, self.nblocks = nbytes, nblocks
+        self.block_off, self.out_off = batch.build_index(ctx, self.meta, nblocks)
+        ctx.sync()
+        for e in enc:
+            e.free()
+
+    def expected(self, sid: int, v0: int, v1: int) -> np.ndarray:
+        """Source values [v0, v1) of stream sid (for checks)."""
+        return self.sources[self.streams[sid].source][v0:v1]
+
+    def free(self) -> None:
+        for a in (self.data, self.meta, self.block_off, self.out_off):
+            a.free()
+
+
+@dataclass
+class Request:
+    kind: int
+    arrival: float = 0.0            # seconds after the start of serving
+    stream: int = -1                # decompress / get
+    first: int = 0                  # decompress: first block within the stream
+    nblocks: int = 0                # decompress: blocks
+    positions: np.ndarray | None = None   # get: value positions within the stream
+    values: np.ndarray | None = None      # compress: uint32 or uint64 values
+    flag: int = 0                   # compress
+    # set by the engine
+    done: float = math.nan
+    tick: int = -1
+    result: np.ndarray | None = None      # values, Get values, or packed bytes
+    status: np.ndarray | None = None      # per-block (decompress) / per-position (get) status words
+    block_len: np.ndarray | None = None   # compress: byte length of each packed block
+
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class _Heap:
+    """Bump allocator over one device allocation, reset every tick."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.mem = DeviceArray(nbytes, np.uint8, ctx)
+        self.cap = nbytes
+        self.top = 0
+
+    def reset(self) -> None:
+        self.top = 0
+
+    def take(self, nbytes: int) -> int:
+        off = (self.top + _ALIGN - 1) // _ALIGN * _ALIGN
+        if off + nbytes > self.cap:
+            raise MemoryError("serving heap exhausted")
+        self.top = off + nbytes
+        return self.mem.ptr + off
+
+
+def _values_of(st: StoreStream, first: int, nblocks: int) -> int:
+    return max(0, min(nblocks * 128, st.nvalues - first * 128))
+
+
+class Engine:
+    """Continuous-batching executor over a Store (one HIP stream)."""
+
+    def __init__(self, ctx: Context, store: Store, heap_bytes: int | None = None, heap_fraction: float = 0.5):
+        if heap_bytes is None:
+            free, _ = mem_info()
+            heap_bytes = int(free * heap_fraction)
+        self.ctx, self.store = ctx, store
+        self.lib = _lib.load()
+        self.heap = _Heap(ctx, heap_bytes)
+
+    # ---- admission ----
+    def cost(self, r: Request) -> int:
+        """Device bytes the request takes in a tick (with alignment slack)."""
+        if r.kind == DECOMPRESS:
+            st = self.store.streams[r.stream]
+            n = r.nblocks
+            return _values_of(st, r.first, n) * (8 if st.is64 else 4) + (n + 1) * 16 + (n + 1) * 4 + 32 + 4 * _ALIGN
+        if r.kind == GET:
+            return r.positions.size * (8 + 8 + 8 + 4) + 4 * _ALIGN
+        v = r.values
+        nb = (v.size + 127) // 128
+        slot = batch.SLOT64 if v.dtype == np.uint64 else batch.SLOT32
+        return v.nbytes + nb * (2 * slot + 8 + 4 + 4 + 16) + 64 + 12 * _ALIGN
+
+    def check_request(self, r: Request) -> None:
+        if r.kind == DECOMPRESS:
+            st = self.store.streams[r.stream]
+            if r.nblocks < 1 or r.first < 0 or r.first + r.nblocks > st.nblocks:
+                raise ValueError("decompress range outside the stream")
+        elif r.kind == GET:
+            st = self.store.streams[r.stream]
+            p = r.positions
+            if p.size == 0 or p.min() < 0 or p.max() >= st.nvalues:
+                raise ValueError("get position outside the stream")
+        elif r.values.size == 0:
+            raise ValueError("compress of no values")
+        if self.cost(r) > self.heap.cap:
+            raise ValueError("request larger than the serving heap")
+
+    # ---- one tick ----
+    def run_batch(self, reqs: list[Request], deliver: bool = True) -> None:
+        """Runs one batch to completion (results copied to the host when
+        deliver is set)."""
+        lib, ctx, store, heap = self.lib, self.ctx, self.store, self.heap
+        heap.reset()
+        pending = []  # (request, kind-specific handles) to deliver after the sync
+        for is64 in (False, True):
+            dec = [r for r in reqs if r.kind == DECOMPRESS and store.streams[r.stream].is64 == is64]
+            if dec:
+                pending += self._decompress(dec, is64)
+            gets = [r for r in reqs if r.kind == GET and store.streams[r.stream].is64 == is64]
+            if gets:
+                pending += self._get(gets, is64)
+            comp = [r for r in reqs if r.kind == COMPRESS and (r.values.dtype == np.uint64) == is64]
+            if comp:
+                pending += self._compress(comp, is64)
+        ctx.sync()
+        if deliver:
+            for fn in pending:
+                fn()
+
+    def _h2d(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a)
+        p = self.heap.take(max(a.nbytes, 1))
+        if a.nbytes:
+            check(self.lib.utl_memcpy_h2d(self.ctx.handle, p, a.ctypes.data, a.nbytes), "h2d")
+        return p
+
+    def _d2h(self, p: int, n: int, dtype) -> np.ndarray:
+        out = np.empty(n, dtype=dtype)
+        if out.nbytes:
+            check(self.lib.utl_memcpy_d2h(self.ctx.handle, out.ctypes.data, p, out.nbytes), "d2h")
+        return out
+
+    def _decompress(self, reqs: list[Request], is64: bool):
+        lib, ctx, store = self.lib, self.ctx, self.store
+        rows = np.empty((len(reqs), 4), dtype=np.uint64)
+        u = e = 0
+        spans = []
+        for i, r in enumerate(reqs):
+            st = store.streams[r.stream]
+            nv = _values_of(st, r.first, r.nblocks)
+            rows[i] = (st.first_block + r.first, r.nblocks, u, e)
+            spans.append((u, nv, e, r.nblocks))
+            u += nv
+            e += r.nblocks + 1
+        item = 8 if is64 else 4
+        d_req = self._h2d(rows.reshape(-1))
+        gboff = self.heap.take(8 * e)
+        gooff = self.heap.take(8 * e)
+        out = self.heap.take(max(u * item, 1))
+        status = self.heap.take(4 * max(e - 1, 1))
+        check(lib.utl_gather_index(ctx.handle, store.block_off.ptr, store.out_off.ptr, store.nblocks, d_req,
+                                   len(reqs), e, gboff, gooff), "utl_gather_index")
+        f = lib.utl_decode_u64 if is64 else lib.utl_decode_u32
+        check(f(ctx.handle, store.data.ptr, gboff, gooff, e - 1, out, status, None), "gathered decode")
+
+        def deliver():
+            vals = self._d2h(out, u, np.uint64 if is64 else np.uint32)
+            sts = self._d2h(status, max(e - 1, 0), np.int32)
+            for r, (u0, nv, e0, nb) in zip(reqs, spans):
+                r.result = vals[u0:u0 + nv].copy()
+                r.status = sts[e0:e0 + nb].copy()
+        return [deliver]
+
+    def _get(self, reqs: list[Request], is64: bool):
+        lib, ctx, store = self.lib, self.ctx, self.store
+        blocks, pos, spans = [], [], []
+        n = 0
+        for r in reqs:
+            st = store.streams[r.stream]
+            p = r.positions.astype(np.int64)
+            blocks.append((st.first_block + p // 128).astype(np.uint64))
+            pos.append(p % 128)
+            spans.append((n, p.size))
+            n += p.size
+        d_blk = self._h2d(np.concatenate(blocks))
+        d_pos = self._h2d(np.concatenate(pos).astype(np.int64))
+        out = self.heap.take(8 * n)
+        status = self.heap.take(4 * n)
+        check(lib.utl_get_batch(ctx.handle, int(is64), store.data.ptr, store.block_off.ptr, store.out_off.ptr,
+                                store.nblocks, d_blk, d_pos, n, out, status), "utl_get_batch")
+
+        def deliver():
+            vals = self._d2h(out, n, np.uint64 if is64 else np.uint32)
+            sts = self._d2h(status, n, np.int32)
+            for r, (a, k) in zip(reqs, spans):
+                r.result = vals[a:a + k].copy()
+                r.status = sts[a:a + k].copy()
+        return [deliver]
+
+    def _compress(self, reqs: list[Request], is64: bool):
+        lib, ctx = self.lib, self.ctx
+        dtype = np.uint64 if is64 else np.uint32
+        lens = np.array([r.values.size for r in reqs], dtype=np.uint64)
+        blk_off = batch.plan_blocks(lens)
+        nb = int(blk_off[-1])
+        val_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+        d_vals = self._h2d(np.concatenate([r.values.astype(dtype, copy=False) for r in reqs]))
+        d_val_off = self._h2d(val_off)
+        d_blk_off = self._h2d(blk_off)
+        d_flags = self._h2d(np.array([r.flag for r in reqs], dtype=np.uint8))
+        in_off = self.heap.take(8 * nb)
+        in_meta = self.heap.take(4 * nb)
+        check(lib.utl_make_jobs(ctx.handle, d_val_off, d_blk_off, d_flags, len(reqs), in_off, in_meta),
+              "utl_make_jobs")
+        slot = batch.SLOT64 if is64 else batch.SLOT32
+        slots = self.heap.take(nb * slot)
+        meta = self.heap.take(4 * nb)
+        check(lib.utl_encode_blocks(ctx.handle, int(is64), d_vals, in_off, in_meta, nb, slots, meta),
+              "utl_encode_blocks")
+        boff = self.heap.take(8 * (nb + 1))
+        ooff = self.heap.take(8 * (nb + 1))
+        check(lib.utl_index_build(ctx.handle, meta, nb, boff, ooff), "utl_index_build")
+        data = self.heap.take(nb * slot + 64)
+        check(lib.utl_compact(ctx.handle, int(is64), slots, boff, nb, data), "utl_compact")
+
+        def deliver():
+            bo = self._d2h(boff, nb + 1, np.uint64)
+            metas = self._d2h(meta, nb, np.uint32)
+            allb = self._d2h(data, int(bo[-1]), np.uint8)
+            for i, r in enumerate(reqs):
+                b0, b1 = int(blk_off[i]), int(blk_off[i + 1])
+                r.result = allb[int(bo[b0]):int(bo[b1])].copy()
+                r.block_len = (metas[b0:b1] & 0xFFFFFF).astype(np.int64)
+        return [deliver]
+
+    # ---- the loop ----
+    def serve(self, requests: list[Request], deliver: bool = True, clock=time.perf_counter) -> dict:
+        """Serves requests by arrival time (seconds from the call). Returns
+        latency and throughput statistics."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            self.check_request(r)
+        t0 = clock()
+        i = tick = 0
+        sizes = []
+        while i < len(reqs):
+            now = clock() - t0
+            if reqs[i].arrival > now:
+                time.sleep(min(reqs[i].arrival - now, 0.001))
+                continue
+            used, b = 0, []
+            while i < len(reqs) and reqs[i].arrival <= now:
+                c = self.cost(reqs[i])
+                if b and used + c > self.heap.cap:
+                    break
+                b.append(reqs[i])
+                used += c
+                i += 1
+            self.run_batch(b, deliver)
+            t = clock() - t0
+            for r in b:
+                r.done, r.tick = t, tick
+            tick += 1
+            sizes.append(len(b))
+        wall = clock() - t0
+        return summarize(reqs, wall, self.store, ticks=tick, mean_batch=float(np.mean(sizes)) if sizes else 0.0)
+
+
+def request_bytes(r: Request, store: Store) -> int:
+    """Bytes a request delivers (decoded values, Get values) or consumes
+    (values to compress)."""
+    if r.kind == DECOMPRESS:
+        st = store.streams[r.stream]
+        return _values_of(st, r.first, r.nblocks) * (8 if st.is64 else 4)
+    if r.kind == GET:
+        return r.positions.size * (8 if store.streams[r.stream].is64 else 4)
+    return r.values.nbytes
+
+
+def summarize(reqs: list[Request], wall: float, store: Store, **extra) -> dict:
+    lat = np.array([r.latency() for r in reqs]) * 1e3
+    out = {"requests": len(reqs), "wall_s": round(wall, 4),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "GBps": round(sum(request_bytes(r, store) for r in reqs) / wall / 1e9, 3)}
+    for k, name in enumerate(KIND_NAMES):
+        lk = np.array([r.latency() for r in reqs if r.kind == k]) * 1e3
+        if lk.size:
+            out[name] = {"n": int(lk.size), "p50_ms": round(float(np.percentile(lk, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lk, 99)), 3)}
+    out.update(extra)
+    return out
+
+
+def make_requests(store: Store, pool: list[tuple[np.ndarray, int]], n: int, rate: float, seed: int = 5,
+                  min_values: int = 128, max_values: int = 4 << 20, mix=(0.6, 0.25, 0.15),
+                  max_get: int = 1024) -> list[Request]:
+    """Poisson arrivals at `rate` per second; kinds by `mix` (decompress,
+    compress, get); sizes log-uniform in [min_values, max_values] values.
+    Decompress reads a random block range of a random stream; Get reads
+    1..max_get random positions; compress takes a random slice of a random
+    (values, flag) pool entry."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    kinds = rng.choice(3, n, p=np.asarray(mix) / np.sum(mix))
+    sizes = np.floor(np.exp(rng.uniform(np.log(min_values), np.log(max_values + 1), n))).astype(np.int64)
+    reqs = []
+    for t, k, sz in zip(arrivals, kinds, sizes):
+        if k == DECOMPRESS:
+            sid = int(rng.integers(len(store.streams)))
+            st = store.streams[sid]
+            nb = int(min(max(1, (sz + 127) // 128), st.nblocks))
+            first = int(rng.integers(0, st.nblocks - nb + 1))
+            reqs.append(Request(DECOMPRESS, float(t), stream=sid, first=first, nblocks=nb))
+        elif k == GET:
+            sid = int(rng.integers(len(store.streams)))
+            st = store.streams[sid]
+            npos = int(rng.integers(1, max_get + 1))
+            reqs.append(Request(GET, float(t), stream=sid, positions=rng.integers(0, st.nvalues, npos)))
+        else:
+            vals, flag = pool[int(rng.integers(len(pool)))]
+            m = int(min(sz, vals.size))
+            a = int(rng.integers(0, vals.size - m + 1))
+            reqs.append(Request(COMPRESS, float(t), values=vals[a:a + m], flag=flag))
+    return reqs
