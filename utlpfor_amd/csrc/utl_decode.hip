@@ -38,11 +38,15 @@ constexpr int kCtrlMax = kWaveBlocks * 32;  // StreamVByte control bytes of a wa
 // are entries [cbase_b, cbase_b + ctrl_b) of tab, with cbase_b the control
 // bytes of the blocks before it, and its exception i is slot 4*cbase_b + i of
 // aux. A block in position-list mode later reuses aux[128*b ..) to scatter.
+constexpr int kAux = kWaveBlocks * kBlock + 4;  // flat exception slots (+ room for reads past the last)
+
 struct alignas(16) WaveLds {
   uint32_t arena[kArena / 4];
-  uint32_t aux[kWaveBlocks * kBlock + 4];  // exception high bits by flat slot (+1 read past the last)
-  uint32_t tab[kCtrlMax];                  // per control byte: data offset | bw << 12 | control byte << 24
+  uint32_t aux[kAux];       // exception high bits by flat slot
+  uint32_t tab[kCtrlMax];   // per control byte: data offset | bw << 12 | control byte << 24
 };
+// 4 x 7696 B per workgroup: 5 workgroups per CU (one more byte of LDS per
+// wave costs a workgroup per CU).
 static_assert(sizeof(WaveLds) == 7696, "LDS per wavefront");
 
 __device__ __forceinline__ void wave_fence() {
@@ -213,14 +217,33 @@ __device__ __forceinline__ int32_t sub_prelude(const uint32_t* a, int at, int64_
   return UTL_OK;
 }
 
-// Bitmap patch of one value pair: the pair's bits are the low two of `bits`;
-// x[rank] is the high part of the first set one (exceptions.go:246-284).
-__device__ __forceinline__ void patch_pair(const uint32_t* x, uint32_t rank, uint32_t bits, uint32_t& va,
+// Bitmap patch of one value pair at bits b, b+1 of bitmap word w, whose
+// lower bits hold `rank` set bits: x[rank] is the high part of the first set
+// one (exceptions.go:246-284).
+__device__ __forceinline__ void patch_pair(const uint32_t* x, uint32_t w, int b, uint32_t rank, uint32_t& va,
                                            uint32_t& vb) {
   const uint32_t lo = x[rank], hi = x[rank + 1];
-  const bool s0 = (bits & 1u) != 0, s1 = (bits & 2u) != 0;
-  va |= s0 ? lo : 0u;
-  vb |= s1 ? (s0 ? hi : lo) : 0u;
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_sbfe((int)w, b, 1);      // 0 or ~0
+  const uint32_t s1 = (uint32_t)__builtin_amdgcn_sbfe((int)w, b + 1, 1);
+  va |= lo & s0;
+  vb |= ((hi & s0) | (lo & ~s0)) & s1;
+}
+
+// One StreamVByte value (svb.go:9-24, exceptions.go:32-88) shifted into the
+// high bits (exceptions.go:83-86, a shift by 32 gives 0): e is the table
+// entry of its control byte (data offset of the byte's first value, bw);
+// the codes of the slots before it (m_lo / m_hi select their low and high
+// bits in the entry) give the offset within.
+__device__ __forceinline__ uint32_t svb_value(const uint32_t* a, uint32_t e, int slot, uint32_t m_lo,
+                                              uint32_t m_hi) {
+  const int pre = slot + __popc(e & m_lo) + 2 * __popc(e & m_hi);
+  const uint32_t code = (e >> (24 + 2 * slot)) & 3;
+  const int off = (int)(e & 0xFFF) + pre;
+  const int q = off >> 2;
+  uint32_t v = __builtin_amdgcn_alignbyte(a[q + 1], a[q], off);  // uses off & 3
+  const uint32_t drop = (code ^ 3) << 3;                          // bytes past the value's length
+  v = (v << drop) >> drop;
+  return (uint32_t)((uint64_t)v << ((e >> 12) & 63));
 }
 
 // Decodes one staged sub-block per 8-lane group whose first byte is at arena
@@ -299,19 +322,21 @@ __device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int
     }
     wave_fence();
     // Slot f belongs to control byte f >> 2, code slot f & 3 (constant per
-    // lane since f advances by 64). Slots past a block's excCount decode
-    // bytes nobody reads.
+    // lane since f advances by 64). Slots past a block's excCount, and the
+    // slots past the table of the last iteration, decode bytes nobody reads.
     const int slot = lane_id() & 3;
-    const uint32_t pre_mask = (1u << (2 * slot)) - 1u;
+    const uint32_t pre_mask = ((1u << (2 * slot)) - 1u) << 24;  // codes of the slots before, in the entry
+    const uint32_t m_lo = pre_mask & 0x55000000u, m_hi = pre_mask & 0xAA000000u;
     const int nslots = 4 * ctot;
 #pragma unroll 1
-    for (int f = lane_id(); f < nslots; f += 64) {
-      const uint32_t e = W->tab[f >> 2];
-      const uint32_t c = e >> 24;
-      const uint32_t low = c & pre_mask;
-      const int pre = slot + (int)(low & 3) + (int)((low >> 2) & 3) + (int)((low >> 4) & 3);
-      const int l = (int)((c >> (2 * slot)) & 3) + 1;
-      W->aux[f] = go_shl32(lds_bytes(a, (int)(e & 0xFFF) + pre, l), (int)((e >> 12) & 63));
+    for (int f = lane_id(); f < nslots; f += 128) {
+      // slot f + 64 may lie past the table in the last iteration: its
+      // indexes are clamped and its value lands in a slot nobody reads
+      const int f1 = f + 64;
+      const uint32_t e0 = W->tab[f >> 2], e1 = W->tab[min(f1 >> 2, kCtrlMax - 1)];
+      const uint32_t x0 = svb_value(a, e0, slot, m_lo, m_hi), x1 = svb_value(a, e1, slot, m_lo, m_hi);
+      W->aux[f] = x0;
+      W->aux[min(f1, kAux - 1)] = x1;
     }
     wave_fence();
   }
@@ -347,8 +372,8 @@ __device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int
         for (int d = 0; d < 4; ++d) {
           const uint32_t w = bm[d];
           const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-          patch_pair(x, pd + __popc(w & ma), w >> ba, v0[2 * d], v1[2 * d]);
-          patch_pair(x, pd + __popc(w & mb), w >> bb, v0[2 * d + 1], v1[2 * d + 1]);
+          patch_pair(x, w, ba, pd + __popc(w & ma), v0[2 * d], v1[2 * d]);
+          patch_pair(x, w, bb, pd + __popc(w & mb), v0[2 * d + 1], v1[2 * d + 1]);
         }
       }
     }
@@ -629,6 +654,12 @@ __device__ __forceinline__ int32_t decode_u64_block(WaveLds* W, bool act, int bl
   return st;
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Decodes the wave's blocks [b0, b0 + nb). The wave-uniform round logic
 // stages as many consecutive blocks as fit the arena, decodes them, and
 // repeats until every block is done.
@@ -642,20 +673,15 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
   const uint64_t my_o0 = boff[b], my_o1 = boff[b + 1];
   const uint64_t out0 = ooff[b], slot_n = ooff[b + 1] - out0;
   const int64_t len = (int64_t)(my_o1 - my_o0);
-  // Common case: the whole group fits the arena in one round. Its offsets
-  // are loaded together (uniform loads) and checked without a loop.
-  uint64_t o[kWaveBlocks + 1];
-#pragma unroll
-  for (int k = 0; k <= kWaveBlocks; ++k) o[k] = boff[b0 + (k < nb ? k : nb)];
-  bool mono = true;
-  uint64_t o_lastm1 = o[0];
-#pragma unroll
-  for (int k = 0; k < kWaveBlocks; ++k) {
-    if (k < nb) mono = mono && o[k + 1] >= o[k];
-    if (k == nb - 1) o_lastm1 = o[k];
-  }
-  const int64_t ext_all = clamp_extent<U64>((int64_t)(o[kWaveBlocks] - o_lastm1));
-  const bool one_round = mono && (int64_t)(o_lastm1 - o[0]) + ext_all + 15 + 4 <= kArena;
+  // Common case: the whole group fits the arena in one round: every block's
+  // offsets are monotone (one ballot over the lanes' own offsets) and the
+  // bytes from the first block's start to the last block's extent fit.
+  const bool mono = __ballot(mine && my_o1 < my_o0) == 0;
+  const int last_lane = kTpb * (nb - 1);
+  const uint64_t o_first = readlane64(my_o0, 0);
+  const uint64_t o_lastm1 = readlane64(my_o0, last_lane);
+  const int64_t ext_all = clamp_extent<U64>((int64_t)(readlane64(my_o1, last_lane) - o_lastm1));
+  const bool one_round = mono && (int64_t)(o_lastm1 - o_first) + ext_all + 15 + 4 <= kArena;
   int first = 0;
 #pragma unroll 1
   while (first < nb) {
@@ -663,7 +689,7 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
     int last;
     int64_t ext_last;
     if (one_round) {
-      base = o[0];
+      base = o_first;
       last = nb;
       lo = o_lastm1;
       ext_last = ext_all;
