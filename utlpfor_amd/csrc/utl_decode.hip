@@ -170,51 +170,41 @@ struct SubPlan {
   uint32_t for_base;
 };
 
-// Header checks, in the reference's order, up to the payload (per thread;
-// the early returns only leave this helper). Returns the status; *go is set
-// when the sub-block has values to decode (count > 0 and every check so far
-// passed). A count of 0 is a valid empty block of 4 bytes.
+// Header checks up to the payload (unpack_block.go:62-172), without branches:
+// every field is read (all reads stay within the staged bytes of the block
+// and the arena) and the status is chosen by the reference's precedence:
+// length < 4, integer type, count 0 (a valid empty block of 4 bytes), count
+// > 128, bw > 32, short for payload + index, short for svbLen. Returns the
+// status; go is set when the sub-block has values to decode.
 template <bool ForU64>
 __device__ __forceinline__ int32_t sub_prelude(const uint32_t* a, int at, int64_t len, SubPlan& p, bool& go,
                                                int& consumed) {
-  go = false;
-  consumed = 0;
-  if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
   p.h = decode_header(lds_u32(a, at));
   const Hdr& h = p.h;
-  bool combine = h.combine;
-  if (ForU64) {
-    if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
-  } else {
-    if (h.int_type != 2 && h.int_type != 1) return UTL_ERR_UNSUPPORTED_TYPE;
-    combine = false;
-  }
-  if ((unsigned)(h.count - 1) >= (unsigned)kBlock || (unsigned)h.bw > 32u) {
-    if (h.count == 0) {
-      consumed = kHdr;
-      return UTL_OK;
-    }
-    if (h.count > kBlock) return UTL_ERR_INVALID_BLOCK_LENGTH;
-    return UTL_ERR_INVALID_BUFFER;
-  }
+  const bool combine = ForU64 && h.combine;  // the u32 API ignores it (unpack_block.go:81)
+  const bool type_ok = ForU64 ? h.int_type == 3 : (h.int_type == 2 || h.int_type == 1);
   const bool has_exc = h.exc > 0;
   const bool has_for = h.for_width > 0;
   p.u64s = ForU64 && is_for64_single(h.int_type, h.for_width, combine);
-  int poff = kHdr + (has_exc ? kSvbLen : 0);
-  const int for_off = poff;
-  if (has_for) poff += p.u64s ? kFor64 : for_base_bytes(h.for_width);
-  if (combine) poff += kB2Len;
+  const int for_off = kHdr + (has_exc ? kSvbLen : 0);
+  const int fbb = for_base_bytes(h.for_width);
+  const int poff = for_off + (has_for ? (p.u64s ? kFor64 : fbb) : 0) + (combine ? kB2Len : 0);
   p.poff = poff;
   const int64_t end = (int64_t)poff + h.bw * 16 + (has_exc ? exc_index_size(h.exc) : 0);
-  if (len < end) return UTL_ERR_INVALID_BUFFER;
-  p.svb_len = 0;
-  if (has_exc) {
-    p.svb_len = (int)lds_bytes(a, at + kHdr, 2);
-    if (end + p.svb_len > len) return UTL_ERR_INVALID_BUFFER;
-  }
-  p.for_base = (has_for && !p.u64s) ? lds_bytes(a, at + for_off, for_base_bytes(h.for_width)) : 0u;
-  go = true;
-  return UTL_OK;
+  const uint32_t svb_raw = lds_bytes(a, at + kHdr, 2);
+  const uint32_t for_raw = lds_u32(a, at + for_off);
+  p.svb_len = has_exc ? (int)svb_raw : 0;
+  p.for_base = (has_for && !p.u64s) ? (fbb >= 4 ? for_raw : (for_raw & ((1u << (8 * fbb)) - 1u))) : 0u;
+  int32_t st = (has_exc && end + p.svb_len > len) ? UTL_ERR_INVALID_BUFFER : UTL_OK;
+  st = len < end ? UTL_ERR_INVALID_BUFFER : st;
+  st = (unsigned)h.bw > 32u ? UTL_ERR_INVALID_BUFFER : st;
+  st = h.count > kBlock ? UTL_ERR_INVALID_BLOCK_LENGTH : st;
+  st = h.count == 0 ? UTL_OK : st;
+  st = type_ok ? st : UTL_ERR_UNSUPPORTED_TYPE;
+  st = len < kHdr ? UTL_ERR_INVALID_BUFFER : st;
+  go = st == UTL_OK && h.count > 0;
+  consumed = st == UTL_OK && h.count == 0 ? kHdr : 0;
+  return st;
 }
 
 // Bitmap patch of one value pair at bits b, b+1 of bitmap word w, whose
@@ -255,7 +245,7 @@ template <bool ForU64>
 __device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int at, int64_t len, int j,
                                               uint32_t* v0, uint32_t* v1, SubResult& res) {
   const uint32_t* a = W->arena;
-  SubPlan p;
+  SubPlan p{};  // fields the prelude does not reach stay 0
   bool go = false;
   int consumed = 0;
   int32_t st = act ? sub_prelude<ForU64>(a, at, len, p, go, consumed) : UTL_OK;
@@ -264,48 +254,41 @@ __device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int
   if (go) unpack_any(p.h.bw, a, at + p.poff, j, v0, v1);
 
   // ---- exception checks and this block's control-byte table (exceptions.go:32-88) ----
-  int ctrl_n = 0;  // control bytes this group contributes to the wave table
-  uint4 te = make_uint4(0, 0, 0, 0);
-  int exc_start = 0;
-  if (go) {
-    exc_start = at + p.poff + p.h.bw * 16;
-    consumed = p.poff + p.h.bw * 16;
-    if (p.h.exc > 0) {
-      const int exc = p.h.exc;
-      const int ctrl = svb_ctrl_count(exc);
-      if (exc > p.h.count || ctrl > p.svb_len) {
-        st = UTL_ERR_INVALID_BUFFER;
-      } else {
-        const int svb_start = exc_start + exc_index_size(exc);
-        // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15).
-        // Only the slots of exceptions < excCount count (exceptions.go:63-80):
-        // data bytes = valid slots + sum of their 2-bit codes, summed per
-        // byte with SWAR arithmetic.
-        const int nv = min(max(exc - 16 * j, 0), 16);  // valid slots in this dword
-        const uint32_t cw = nv > 0 ? lds_u32(a, svb_start + 4 * j) : 0u;
-        const uint32_t cm = cw & (nv >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nv)) - 1u));
-        const uint32_t t2 = (cm & 0x33333333u) + ((cm >> 2) & 0x33333333u);
-        const uint32_t t4 = (t2 & 0x0F0F0F0Fu) + ((t2 >> 4) & 0x0F0F0F0Fu);  // code sum per byte
-        const uint32_t pre = t4 * 0x01010100u;                                // exclusive byte prefix
-        const int part = nv + (int)((t4 * 0x01010101u) >> 24);
-        const uint32_t incl = grp_incl_scan((uint32_t)part, j);
-        const int total = (int)grp_last(incl);
-        if (ctrl + total > p.svb_len) {
-          st = UTL_ERR_INVALID_BUFFER;
-        } else {
-          // Data offset of each owned control byte, with its bw and the byte.
-          const uint32_t off = (uint32_t)(svb_start + ctrl + (int)incl - part);
-          const uint32_t bwf = (uint32_t)p.h.bw << 12;
-          te.x = (off + ((pre >> 0) & 0xFF)) | bwf | ((cw & 0xFFu) << 24);
-          te.y = (off + min(4, nv) + ((pre >> 8) & 0xFF)) | bwf | (((cw >> 8) & 0xFFu) << 24);
-          te.z = (off + min(8, nv) + ((pre >> 16) & 0xFF)) | bwf | (((cw >> 16) & 0xFFu) << 24);
-          te.w = (off + min(12, nv) + (pre >> 24)) | bwf | ((cw >> 24) << 24);
-          ctrl_n = ctrl;
-          consumed = svb_start + p.svb_len - at;
-        }
-      }
-    }
-  }
+  // Computed without branches by every thread; a group contributes control
+  // bytes to the wave table (ctrl_n) only when all its checks pass. The
+  // three checks (excCount > count, control bytes > svbLen, control sums >
+  // svbLen) all give ErrInvalidBuffer, so their order does not matter.
+  const int exc = go ? p.h.exc : 0;
+  const int exc_start = at + p.poff + p.h.bw * 16;
+  if (go) consumed = p.poff + p.h.bw * 16;
+  const int ctrl = svb_ctrl_count(exc);
+  const int svb_start = exc_start + exc_index_size(exc);
+  // Thread j owns control bytes 4j .. 4j+3 (exceptions 16j .. 16j+15). Only
+  // the slots of exceptions < excCount count (exceptions.go:63-80): data
+  // bytes = valid slots + sum of their 2-bit codes, summed per byte with
+  // SWAR arithmetic.
+  const int nv = min(max(exc - 16 * j, 0), 16);  // valid slots in this dword
+  const uint32_t cw = lds_u32(a, nv > 0 ? svb_start + 4 * j : 0);
+  const uint32_t cm = cw & (nv >= 16 ? 0xFFFFFFFFu : ((1u << (2 * nv)) - 1u));
+  const uint32_t t2 = (cm & 0x33333333u) + ((cm >> 2) & 0x33333333u);
+  const uint32_t t4 = (t2 & 0x0F0F0F0Fu) + ((t2 >> 4) & 0x0F0F0F0Fu);  // code sum per byte
+  const uint32_t pre = t4 * 0x01010100u;                                // exclusive byte prefix
+  const int part = nv + (int)((t4 * 0x01010101u) >> 24);
+  const uint32_t incl = grp_incl_scan((uint32_t)part, j);
+  const int total = (int)grp_last(incl);
+  const bool svb_ok = exc <= p.h.count && ctrl + total <= p.svb_len;
+  if (exc > 0 && !svb_ok) st = UTL_ERR_INVALID_BUFFER;
+  const bool has_svb = exc > 0 && svb_ok;
+  const int ctrl_n = has_svb ? ctrl : 0;  // control bytes this group contributes to the wave table
+  if (has_svb) consumed = svb_start + p.svb_len - at;
+  // Data offset of each owned control byte, with its bw and the byte.
+  const uint32_t off = (uint32_t)(svb_start + ctrl + (int)incl - part);
+  const uint32_t bwf = (uint32_t)p.h.bw << 12;
+  uint4 te;
+  te.x = (off + ((pre >> 0) & 0xFF)) | bwf | ((cw & 0xFFu) << 24);
+  te.y = (off + min(4, nv) + ((pre >> 8) & 0xFF)) | bwf | (((cw >> 8) & 0xFFu) << 24);
+  te.z = (off + min(8, nv) + ((pre >> 16) & 0xFF)) | bwf | (((cw >> 16) & 0xFFu) << 24);
+  te.w = (off + min(12, nv) + (pre >> 24)) | bwf | ((cw >> 24) << 24);
 
   // ---- wave-wide StreamVByte decode of every group's exceptions ----
   const uint32_t cincl = wave_group_scan((uint32_t)ctrl_n);
