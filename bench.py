@@ -104,7 +104,7 @@ def main() -> int:
         if args.workload == "plain1m":
             print(json.dumps(runbench.plain_1m(ctx)), flush=True)
         elif args.workload == "uneven":
-            print(json.dumps(runbench.uneven(ctx, nstreams=args.streams)), flush=True)
+            print(json.dumps(runbench.uneven(ctx, nstreams=args.streams, progress=log)), flush=True)
         elif args.workload == "u64":
             for rec in runbench.u64_and_get(ctx):
                 print(json.dumps(rec), flush=True)
