@@ -314,32 +314,69 @@ __global__ __launch_bounds__(64 * kEncWaves) void k_encode_u64(const uint64_t* _
   if (t == 0) out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
 }
 
-// Concatenates the slots: block b's bytes go to dst + boff[b].
+// Little-endian dword of slot bytes [i, i + 4) (slots are 4-byte aligned and
+// i + 8 stays inside the slot for every i the compaction asks for).
+__device__ __forceinline__ uint32_t slot_u32(const uint8_t* __restrict__ s, uint32_t i) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(s) + (i >> 2);
+  return __builtin_amdgcn_alignbyte(w[1], w[0], i & 3);
+}
+
+// Concatenates the slots into dst (block b's bytes at dst + boff[b]) with
+// aligned dword stores: the output dword d is written by the block holding
+// byte 4d; blocks are at least 4 bytes, so its bytes past that block come
+// from the next block's slot. One wavefront per block. dst must have room
+// for the dword holding the last byte.
 __global__ __launch_bounds__(256) void k_compact(const uint8_t* __restrict__ slots, uint32_t slot_bytes,
                                                  const uint64_t* __restrict__ boff, uint64_t nblocks,
                                                  uint8_t* __restrict__ dst) {
   const uint64_t b = (uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6);
   if (b >= nblocks) return;
-  const uint64_t o = boff[b];
-  const int len = (int)(boff[b + 1] - o);
+  const uint64_t o = boff[b], o1 = boff[b + 1];
   const uint8_t* s = slots + b * slot_bytes;
-  for (int i = lane_id(); i < len; i += 64) dst[o + i] = s[i];
+  const uint8_t* sn = slots + (b + 1 < nblocks ? b + 1 : b) * slot_bytes;
+  uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
+  const uint64_t d0 = (o + 3) >> 2, d1 = (o1 + 3) >> 2;  // dwords whose first byte is in this block
+  for (uint64_t d = d0 + lane_id(); d < d1; d += 64) {
+    const uint64_t p = 4 * d;            // first byte of the dword, in [o, o1)
+    // in <= MaxBlockLength - 1 for every block the encoders produce; the
+    // clamp keeps reads inside the slot for an index built over rejected
+    // blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
+    const uint32_t in = (uint32_t)min(p - o, (uint64_t)(slot_bytes - 8));
+    uint32_t w = slot_u32(s, in);
+    const uint64_t mine = o1 - p;        // bytes of the dword in this block (>= 1)
+    if (mine < 4) {
+      const uint32_t keep = (1u << (8 * mine)) - 1u;
+      w = (w & keep) | (b + 1 < nblocks ? (slot_u32(sn, 0) << (8 * mine)) : 0u);
+    }
+    d32[d] = w;
+  }
 }
 
 // Job table for streams cut into 128-value blocks: stream s covers values
 // [val_off[s], val_off[s+1]) with flag flags[s]; its blocks are numbered from
-// blk_off[s]. One thread per block.
-__global__ void k_make_jobs(const uint64_t* __restrict__ val_off, const uint64_t* __restrict__ blk_off,
-                            const uint8_t* __restrict__ flags, uint64_t nstreams, uint64_t* __restrict__ in_off,
-                            uint32_t* __restrict__ in_meta) {
-  const uint64_t s = (uint64_t)blockIdx.y * 65535ull + blockIdx.x;
-  if (s >= nstreams) return;
-  const uint64_t b0 = blk_off[s], nb = blk_off[s + 1] - b0;
-  const uint64_t v0 = val_off[s], nv = val_off[s + 1] - v0;
-  for (uint64_t k = threadIdx.x; k < nb; k += blockDim.x) {
-    const uint64_t cnt = (nv - 128 * k) < 128 ? (nv - 128 * k) : 128;
-    in_off[b0 + k] = v0 + 128 * k;
-    in_meta[b0 + k] = (uint32_t)cnt | ((uint32_t)flags[s] << 8);
+// blk_off[s] (blk_off[nstreams] blocks in all, read on the device). A
+// grid-stride loop over blocks; each block's stream by binary search (the
+// last stream starting at or before it, so empty streams are skipped).
+__global__ __launch_bounds__(256) void k_make_jobs(const uint64_t* __restrict__ val_off,
+                                                   const uint64_t* __restrict__ blk_off,
+                                                   const uint8_t* __restrict__ flags, uint64_t nstreams,
+                                                   uint64_t* __restrict__ in_off, uint32_t* __restrict__ in_meta) {
+  const uint64_t nblocks = blk_off[nstreams];
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nblocks; k += step) {
+    uint64_t lo = 0, hi = nstreams - 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) >> 1;
+      if (blk_off[mid] <= k)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint64_t s = lo, i = k - blk_off[s];
+    const uint64_t v0 = val_off[s], nv = val_off[s + 1] - v0;
+    const uint64_t rest = nv - 128 * i;
+    in_off[k] = v0 + 128 * i;
+    in_meta[k] = (uint32_t)(rest < 128 ? rest : 128) | ((uint32_t)flags[s] << 8);
   }
 }
 
@@ -377,9 +414,6 @@ hipError_t utl_launch_compact(const uint8_t* slots, uint32_t slot_bytes, const u
 hipError_t utl_launch_make_jobs(const uint64_t* val_off, const uint64_t* blk_off, const uint8_t* flags,
                                 uint64_t nstreams, uint64_t* in_off, uint32_t* in_meta, hipStream_t stream) {
   if (nstreams == 0) return hipSuccess;
-  const uint64_t gx = nstreams < 65535 ? nstreams : 65535;
-  const uint64_t gy = (nstreams + 65534) / 65535;
-  k_make_jobs<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream>>>(val_off, blk_off, flags, nstreams, in_off,
-                                                                           in_meta);
+  k_make_jobs<<<dim3(2048), dim3(256), 0, stream>>>(val_off, blk_off, flags, nstreams, in_off, in_meta);
   return hipGetLastError();
 }
