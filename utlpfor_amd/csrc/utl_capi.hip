@@ -351,7 +351,12 @@ static int pack_call(utl_ctx* ctx, bool u64, uint32_t flag, const void* values, 
   const size_t len = meta & 0xFFFFFF;
   if (out_len) *out_len = len;
   if (len > dst_cap) return UTL_ERR_ARGUMENT;
-  memcpy(dst, c->h_stage + kResultOff + 1024, len);
+  // the block starts after the zero bytes (0..3) the encoder puts before it
+  // in the slot; its first byte, the count, is not 0
+  const uint8_t* blk = c->h_stage + kResultOff + 1024;
+  int sh = 0;
+  while (sh < 3 && blk[sh] == 0) ++sh;
+  memcpy(dst, blk + sh, len);
   if (inplace && !(flag & UTL_NOINPLACE)) memcpy(inplace, c->h_stage + kResultOff + 64, n * 4);
   return UTL_OK;
 }

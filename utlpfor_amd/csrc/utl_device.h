@@ -120,4 +120,49 @@ __device__ __forceinline__ uint64_t wave_reduce_or64(uint64_t x) {
 
 __device__ __forceinline__ int bit_len32(uint32_t x) { return x ? 32 - __clz((int)x) : 0; }
 
+// ---- reductions and scans within aligned groups of 8 lanes (DPP) ----
+// Every lane of a group must be active; groups never read other groups.
+
+// value of the lane given by a DPP control: quad_perm [1,0,3,2] (0xB1),
+// quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141, lane i <- lane 7-i of
+// its 8 lanes)
+template <int Ctrl>
+__device__ __forceinline__ uint32_t g8_dpp(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, Ctrl, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t g8_or(uint32_t x) {
+  x |= g8_dpp<0xB1>(x);
+  x |= g8_dpp<0x4E>(x);
+  return x | g8_dpp<0x141>(x);
+}
+__device__ __forceinline__ uint32_t g8_add(uint32_t x) {
+  x += g8_dpp<0xB1>(x);
+  x += g8_dpp<0x4E>(x);
+  return x + g8_dpp<0x141>(x);
+}
+__device__ __forceinline__ uint32_t g8_min(uint32_t x) {
+  x = min(x, g8_dpp<0xB1>(x));
+  x = min(x, g8_dpp<0x4E>(x));
+  return min(x, g8_dpp<0x141>(x));
+}
+__device__ __forceinline__ uint32_t g8_max(uint32_t x) {
+  x = max(x, g8_dpp<0xB1>(x));
+  x = max(x, g8_dpp<0x4E>(x));
+  return max(x, g8_dpp<0x141>(x));
+}
+
+// Inclusive prefix sum over the group (lane j = lane & 7): row_shr:1,2,4
+// within 16-lane rows, keeping only sources of the same group.
+__device__ __forceinline__ uint32_t g8_incl_scan(uint32_t x) {
+  const int j = lane_id() & 7;
+  uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);
+  x += j >= 1 ? y : 0u;
+  y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);
+  x += j >= 2 ? y : 0u;
+  y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);
+  x += j >= 4 ? y : 0u;
+  return x;
+}
+
 }  // namespace utl

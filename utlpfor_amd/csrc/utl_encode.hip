@@ -231,36 +231,371 @@ __device__ __forceinline__ void copy_out(const EncLds* L, int len, uint8_t* __re
   for (int i = lane_id(); i < len; i += 64) dst[i] = L->blk[i];
 }
 
-// Encodes block b of a u32 job table into its slot.
-__global__ __launch_bounds__(64 * kEncWaves) void k_encode_u32(const uint32_t* __restrict__ values,
-                                                              const uint64_t* __restrict__ in_off,
-                                                              const uint32_t* __restrict__ in_meta, uint64_t nblocks,
-                                                              uint8_t* __restrict__ slots, uint32_t slot_bytes,
-                                                              uint32_t* __restrict__ out_meta,
-                                                              uint32_t* __restrict__ work_out) {
-  __shared__ EncLds lds[kEncWaves];
+// ---- uint32 blocks, eight per wavefront ----
+//
+// Thread j (0..7) of a block's 8-lane group owns UTL lanes 2j and 2j+1, i.e.
+// the 16 values at positions 16r + 2j + e, so per-lane delta runs in
+// registers and the payload words of its two lanes are packed from
+// registers. Every per-block decision is a group reduction (DPP), so the
+// decision work of 8 blocks runs as vector instructions in one wavefront.
+//
+// Slot layout: the block starts at slot byte sh = (-poff) & 3, which puts
+// its payload (and everything after it) on a 4-byte boundary, so all stores
+// are aligned dwords. The bytes before the block are 0 and the block's first
+// byte (its count) is not, which is how the compaction finds sh.
+
+constexpr int kEncGrpWaves = 4;
+constexpr int kWaveBlocksEnc = 8;  // blocks per wavefront
+constexpr int kEncTail = 576;      // exception index (<= 16) + control (<= 32) + data (<= 512), rounded up
+
+struct alignas(16) EncGrpLds {
+  uint8_t tail[kWaveBlocksEnc][kEncTail];  // exception index, control and data bytes of each block
+  uint16_t off[kWaveBlocksEnc][kBlock];    // data offset of the exception of each rank
+  uint8_t len[kWaveBlocksEnc][kBlock];     // data bytes of the exception of each rank
+};
+static_assert(sizeof(EncGrpLds) * kEncGrpWaves <= 32768, "LDS per workgroup");
+
+__device__ __forceinline__ uint32_t step_len(uint32_t x) { return (uint32_t)(bit_len32(x) + 3) >> 2; }
+
+// Per-block counts exc[si] of values whose step index exceeds si, si = 0..7
+// (select_bitwidth.go:41-51), as 8-bit fields: si 0..3 in lo, 4..7 in hi.
+// Values at invalid positions must be 0. Each thread packs the step indices
+// of its 16 values into nibbles; nibble s + (7 - si) has bit 3 set exactly
+// when s > si (s <= 8, so no nibble carries).
+__device__ __forceinline__ void step_counts(const uint32_t* x0, const uint32_t* x1, uint32_t& lo, uint32_t& hi) {
+  uint32_t s_a = 0, s_b = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    s_a |= (step_len(x0[r]) << (8 * r)) | (step_len(x1[r]) << (8 * r + 4));
+    s_b |= (step_len(x0[r + 4]) << (8 * r)) | (step_len(x1[r + 4]) << (8 * r + 4));
+  }
+  uint32_t c_lo = 0, c_hi = 0;
+#pragma unroll
+  for (int si = 0; si < 8; ++si) {
+    const uint32_t add = 0x77777777u - (uint32_t)si * 0x11111111u;
+    const uint32_t c = (uint32_t)(__popc((s_a + add) & 0x88888888u) + __popc((s_b + add) & 0x88888888u));
+    if (si < 4)
+      c_lo |= c << (8 * si);
+    else
+      c_hi |= c << (8 * (si - 4));
+  }
+  lo = g8_add(c_lo);  // fields stay <= 128
+  hi = g8_add(c_hi);
+}
+
+// select_bitwidth.go:56-72 chooseBestFromHist over the counts above: the
+// candidate widths below max_si from the top, strict '<' (ties keep the wider
+// width); bytes per exception value (k + 1) >> 1 for k = max_si - si steps.
+__device__ __forceinline__ void choose_best_g(uint32_t lo, uint32_t hi, int max_si, int& best_w, int& best_exc) {
+  int cost = kHdr + (max_si << 6);
+  best_w = max_si << 2;
+  best_exc = 0;
+#pragma unroll
+  for (int si = 7; si >= 0; --si) {
+    const int e = (int)(((si < 4 ? lo : hi) >> (8 * (si & 3))) & 0xFF);
+    const int k = max_si - si;
+    const int c = kHdr + (si << 6) + kSvbLen + (e < kExcBitmap ? e : kExcBitmap) + ((e + 3) >> 2) + e * ((k + 1) >> 1);
+    if (si < max_si && c < cost) {
+      cost = c;
+      best_w = si << 2;
+      best_exc = e;
+    }
+  }
+}
+
+// UTL pack of lanes 2j, 2j+1 (scalar.go:7-59): lane L's word w goes to
+// payload dword 16w + L. v0/v1 hold values already masked to BW bits.
+template <int BW>
+__device__ __forceinline__ void pack_bw(const uint32_t* v0, const uint32_t* v1, uint32_t* __restrict__ dst, int j) {
+  if constexpr (BW > 0) {
+    constexpr int NW = BW / 4;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      uint32_t a = 0, c = 0;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int s = r * BW - 32 * w;
+        if (s > -BW && s < 32) {
+          a |= s >= 0 ? (v0[r] << s) : (v0[r] >> (-s));
+          c |= s >= 0 ? (v1[r] << s) : (v1[r] >> (-s));
+        }
+      }
+      dst[16 * w + 2 * j] = a;
+      dst[16 * w + 2 * j + 1] = c;
+    }
+  }
+}
+
+__device__ __forceinline__ void pack_any(int bw, const uint32_t* v0, const uint32_t* v1, uint32_t* __restrict__ dst,
+                                         int j) {
+  switch (bw) {
+    case 4: pack_bw<4>(v0, v1, dst, j); break;
+    case 8: pack_bw<8>(v0, v1, dst, j); break;
+    case 12: pack_bw<12>(v0, v1, dst, j); break;
+    case 16: pack_bw<16>(v0, v1, dst, j); break;
+    case 20: pack_bw<20>(v0, v1, dst, j); break;
+    case 24: pack_bw<24>(v0, v1, dst, j); break;
+    case 28: pack_bw<28>(v0, v1, dst, j); break;
+    case 32: pack_bw<32>(v0, v1, dst, j); break;
+    default: break;
+  }
+}
+
+// Encodes blocks [8w, 8w + 8) of a u32 job table (wavefront w) into their
+// slots, following packBlockScalar (scalar.go:126-235): FOR decision
+// (for.go:95-118), per-lane delta with zigzag of all values on any decrease
+// (delta.go:23-47), bit width by the cost model (select_bitwidth.go:41-121),
+// UTL payload, exceptions as a sorted position list or a bitmap with their
+// high bits StreamVByte-encoded (exceptions.go:138-186).
+__global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t* __restrict__ values,
+                                                                  const uint64_t* __restrict__ in_off,
+                                                                  const uint32_t* __restrict__ in_meta,
+                                                                  uint64_t nblocks, uint8_t* __restrict__ slots,
+                                                                  uint32_t slot_bytes, uint32_t* __restrict__ out_meta,
+                                                                  uint32_t* __restrict__ work_out) {
+  __shared__ EncGrpLds lds[kEncGrpWaves];
   const uint32_t wave = uniform(threadIdx.x >> 6);
-  const uint64_t b = (uint64_t)blockIdx.x * kEncWaves + wave;
-  if (b >= nblocks) return;
-  EncLds* L = &lds[wave];
+  const uint64_t b0 = ((uint64_t)blockIdx.x * kEncGrpWaves + wave) * kWaveBlocksEnc;
+  if (b0 >= nblocks) return;
+  EncGrpLds* L = &lds[wave];
+  const int blk = lane_id() >> 3, j = lane_id() & 7;
+  const bool mine = b0 + blk < nblocks;
+  const uint64_t b = mine ? b0 + blk : b0;
   const uint32_t meta = in_meta[b];
-  const int n = (int)(meta & 0xFF);
   const uint32_t flag = (meta >> 8) & 0xFF;
-  const int t = lane_id();
-  if (n < 1 || n > kBlock) {  // packBlockScalar: ErrInvalidBuffer (scalar.go:127-129)
-    if (t == 0) out_meta[b] = 0xFFFFFFFFu;
-    return;
+  const int n_req = (int)(meta & 0xFF);
+  const bool ok = mine && n_req >= 1 && n_req <= kBlock;  // else ErrInvalidBuffer (scalar.go:127-129)
+  const int n = ok ? n_req : 0;
+  const uint64_t voff = in_off[b];
+  const uint32_t* v = values + voff;
+
+  uint32_t x0[kRows], x1[kRows];
+  bool ok0[kRows], ok1[kRows];
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = 16 * r + 2 * j;
+    ok0[r] = p < n;
+    ok1[r] = p + 1 < n;
+    x0[r] = ok0[r] ? v[p] : 0u;
+    x1[r] = ok1[r] ? v[p + 1] : 0u;
+    if (ok0[r]) {
+      mn = min(mn, x0[r]);
+      mx = max(mx, x0[r]);
+    }
+    if (ok1[r]) {
+      mn = min(mn, x1[r]);
+      mx = max(mx, x1[r]);
+    }
   }
-  const uint32_t* v = values + in_off[b];
-  uint32_t x0 = (2 * t < n) ? v[2 * t] : 0u;
-  uint32_t x1 = (2 * t + 1 < n) ? v[2 * t + 1] : 0u;
-  const int len = pack_sub_block(L, 0, flag, x0, x1, n, kTypeU32, false, false, 0);
-  copy_out(L, len, slots + b * slot_bytes);
-  if (work_out) {
-    if (2 * t < n) work_out[in_off[b] + 2 * t] = x0;
-    if (2 * t + 1 < n) work_out[in_off[b] + 2 * t + 1] = x1;
+  uint32_t hflags = kTypeU32 | ((flag & UTL_SPECIAL) << 14);
+
+  // ---- FOR (for.go:95-118) ----
+  int fw = 0;
+  uint32_t base = 0;
+  mn = g8_min(mn);
+  mx = g8_max(mx);
+  if (!(flag & UTL_NOFOR) && mn != 0) {
+    const int for_max = ((bit_len32(mx - mn) + 3) >> 2) << 2;
+    const int raw_max = ((bit_len32(mx) + 3) >> 2) << 2;
+    if (for_max < raw_max) {
+      uint32_t lo, hi;
+      step_counts(x0, x1, lo, hi);
+      int std_w, unused;
+      choose_best_g(lo, hi, (int)step_len(mx), std_w, unused);
+      if (for_max < std_w) {
+        base = mn;
+        const int bytes = (bit_len32(mn) + 7) >> 3;
+        fw = bytes == 0 ? 0 : bytes == 1 ? 1 : bytes == 2 ? 2 : 3;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          x0[r] = ok0[r] ? x0[r] - base : 0u;
+          x1[r] = ok1[r] ? x1[r] - base : 0u;
+        }
+        hflags |= (uint32_t)fw << 15;
+      }
+    }
   }
-  if (t == 0) out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
+  // ---- per-lane delta (delta.go:23-47) ----
+  if (flag & UTL_DELTA) {
+    bool dec = false;
+#pragma unroll
+    for (int r = kRows - 1; r >= 1; --r) {
+      dec |= ok0[r] && x0[r] < x0[r - 1];
+      dec |= ok1[r] && x1[r] < x1[r - 1];
+      x0[r] = ok0[r] ? x0[r] - x0[r - 1] : 0u;
+      x1[r] = ok1[r] ? x1[r] - x1[r - 1] : 0u;
+    }
+    if (g8_or(dec ? 1u : 0u)) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        x0[r] = zz_enc(x0[r]);  // zz_enc(0) = 0 keeps invalid positions at 0
+        x1[r] = zz_enc(x1[r]);
+      }
+      hflags |= kHZigZag;
+    }
+    hflags |= kHDelta;
+  }
+  // ---- bit width (select_bitwidth.go:20-121) ----
+  uint32_t orall = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) orall |= x0[r] | x1[r];
+  const int max_si = (int)step_len(g8_or(orall));
+  int bw = max_si << 2, exc = 0;
+  if (!(flag & UTL_NOPATCH)) {
+    uint32_t lo, hi;
+    step_counts(x0, x1, lo, hi);
+    choose_best_g(lo, hi, max_si, bw, exc);
+  }
+  if (work_out && ok) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      if (ok0[r]) work_out[voff + 16 * r + 2 * j] = x0[r];
+      if (ok1[r]) work_out[voff + 16 * r + 2 * j + 1] = x1[r];
+    }
+  }
+
+  const bool has_exc = exc > 0;
+  const int fbb = for_base_bytes(fw);
+  const int poff = kHdr + (has_exc ? kSvbLen : 0) + fbb;
+  const int sh = (-poff) & 3;
+  uint8_t* slot = slots + b * slot_bytes;
+  uint32_t* pay = reinterpret_cast<uint32_t*>(slot + sh + poff);  // 4-byte aligned
+  const uint32_t m = low_mask(bw);
+
+  // ---- payload ----
+  if (ok) {
+    uint32_t p0[kRows], p1[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      p0[r] = x0[r] & m;
+      p1[r] = x1[r] & m;
+    }
+    pack_any(bw, p0, p1, pay, j);
+  }
+
+  // ---- exceptions (exceptions.go:138-186) ----
+  int svb_len = 0;
+  const int isz = exc_index_size(exc);
+  if (has_exc) {
+    // bitmap of exception positions (word d: rows 2d, 2d+1) and ranks
+    uint32_t bm[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int bit = 16 * (r & 1) + 2 * j;
+      bm[r >> 1] |= ((ok0[r] && x0[r] > m) ? 1u : 0u) << bit;
+      bm[r >> 1] |= ((ok1[r] && x1[r] > m) ? 1u : 0u) << (bit + 1);
+    }
+#pragma unroll
+    for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d]);
+    const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
+    uint8_t* lens = L->len[blk];
+    // data bytes of each exception, by rank
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
+      const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
+      const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
+      const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
+      if (e0) lens[rank0] = (uint8_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1);
+      if (e1) lens[rank0 + (e0 ? 1 : 0)] = (uint8_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1);
+    }
+    lds_fence();
+    // thread j: data offsets and control bytes of ranks 16j .. 16j+15
+    const int nvalid = min(max(exc - 16 * j, 0), 16);
+    const uint4 lq = *reinterpret_cast<const uint4*>(&lens[16 * j]);
+    uint32_t lw[4] = {lq.x, lq.y, lq.z, lq.w};
+    uint32_t tot = 0;
+    uint32_t incl4[4], ctrlb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int kv = min(max(nvalid - 4 * k, 0), 4);  // valid ranks in this dword
+      const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
+      const uint32_t l = lw[k] & keep;
+      // codes: length - 1 in 2-bit fields, only for valid ranks
+      const uint32_t c = ((l | ~keep) - 0x01010101u) & keep;  // bytes 0..3
+      ctrlb[k] = (c & 3u) | ((c >> 6) & 0xCu) | ((c >> 12) & 0x30u) | ((c >> 18) & 0xC0u);
+      incl4[k] = l * 0x01010101u + (tot * 0x01010101u);  // inclusive byte prefix + earlier dwords
+      tot += (l * 0x01010101u) >> 24;
+    }
+    const uint32_t scan = g8_incl_scan(tot);
+    const uint32_t tbase = scan - tot;
+    const uint32_t total = (uint32_t)__shfl((int)scan, lane_id() | 7, 64);
+    const int ctrl = svb_ctrl_count(exc);
+    uint16_t* offs = L->off[blk];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int kv = min(max(nvalid - 4 * k, 0), 4);
+      const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
+      const uint32_t l = lw[k] & keep;
+      const uint32_t excl = incl4[k] - l;  // exclusive byte prefix (bytes stay <= 64)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) offs[16 * j + 4 * k + s] = (uint16_t)(tbase + ((excl >> (8 * s)) & 0xFF));
+      if (4 * j + k < ctrl) L->tail[blk][isz + 4 * j + k] = (uint8_t)ctrlb[k];
+    }
+    lds_fence();
+    // exception data (high bits, little endian) and the position index
+    uint8_t* tl = L->tail[blk];
+    const int dbase = isz + ctrl;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
+      const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
+      const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
+      const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
+      const uint32_t rank1 = rank0 + (e0 ? 1u : 0u);
+      if (e0) {
+        const uint32_t h = x0[r] >> bw;
+        const int o = dbase + offs[rank0], l = lens[rank0];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
+        if (exc <= kExcBitmap) tl[rank0] = (uint8_t)(16 * r + 2 * j);
+      }
+      if (e1) {
+        const uint32_t h = x1[r] >> bw;
+        const int o = dbase + offs[rank1], l = lens[rank1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
+        if (exc <= kExcBitmap) tl[rank1] = (uint8_t)(16 * r + 2 * j + 1);
+      }
+    }
+    if (exc > kExcBitmap) {  // bitmap bytes 2j, 2j+1 (position 8q + i is bit i of byte q)
+      tl[2 * j] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1)));
+      tl[2 * j + 1] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1) + 8));
+    }
+    svb_len = ctrl + (int)total;
+    lds_fence();
+    // copy the tail (aligned in the slot right after the payload)
+    if (ok) {
+      uint32_t* dst = pay + 4 * bw;
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(tl);
+      const int ndw = (isz + svb_len + 3) >> 2;
+      for (int k = j; k < ndw; k += 8) dst[k] = src[k];
+    }
+  }
+  const int len = poff + 16 * bw + (has_exc ? isz + svb_len : 0);
+  if (j == 0 && mine) {
+    if (ok) {
+      // bytes [0, poff): header, svbLen, FOR base, written as the slot's
+      // first (sh + poff) / 4 dwords with zeros before the block
+      const uint32_t hw = encode_header(n, bw, exc, hflags);
+      const uint32_t w1 = has_exc ? ((uint32_t)svb_len | (base << 16)) : base;
+      const uint32_t w2 = has_exc ? (fbb > 2 ? base >> 16 : 0u) : 0u;
+      uint32_t* s32 = reinterpret_cast<uint32_t*>(slot);
+      const uint32_t o0 = sh ? (hw << (8 * sh)) : hw;
+      const uint32_t o1 = sh ? __builtin_amdgcn_alignbyte(w1, hw, 4 - sh) : w1;
+      const uint32_t o2 = sh ? __builtin_amdgcn_alignbyte(w2, w1, 4 - sh) : w2;
+      const int nd = (sh + poff) >> 2;
+      s32[0] = o0;
+      if (nd > 1) s32[1] = o1;
+      if (nd > 2) s32[2] = o2;
+      out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
+    } else {
+      out_meta[b] = 0xFFFFFFFFu;
+    }
+  }
 }
 
 // uint64.go:302-333 packUint64Scalar
@@ -314,39 +649,50 @@ __global__ __launch_bounds__(64 * kEncWaves) void k_encode_u64(const uint64_t* _
   if (t == 0) out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
 }
 
-// Little-endian dword of slot bytes [i, i + 4) (slots are 4-byte aligned and
-// i + 8 stays inside the slot for every i the compaction asks for).
+// Little-endian dword of slot bytes [i, i + 4) (slots are 4-byte aligned; the
+// second dword is read only when the bytes straddle it, so a read ending at
+// the slot's last byte stays inside the slot).
 __device__ __forceinline__ uint32_t slot_u32(const uint8_t* __restrict__ s, uint32_t i) {
   const uint32_t* w = reinterpret_cast<const uint32_t*>(s) + (i >> 2);
-  return __builtin_amdgcn_alignbyte(w[1], w[0], i & 3);
+  const uint32_t hi = (i & 3) ? w[1] : 0u;
+  return __builtin_amdgcn_alignbyte(hi, w[0], i & 3);
+}
+
+// Where a block starts in its slot: after the zero bytes the encoders put
+// before it (0..3); a block's first byte, its count, is never 0.
+__device__ __forceinline__ uint32_t slot_shift(const uint8_t* __restrict__ s) {
+  const uint32_t w = *reinterpret_cast<const uint32_t*>(s);
+  return w ? min((uint32_t)__builtin_ctz(w) >> 3, 3u) : 3u;
 }
 
 // Concatenates the slots into dst (block b's bytes at dst + boff[b]) with
 // aligned dword stores: the output dword d is written by the block holding
 // byte 4d; blocks are at least 4 bytes, so its bytes past that block come
-// from the next block's slot. One wavefront per block. dst must have room
-// for the dword holding the last byte.
+// from the next block's slot. Eight lanes per block, eight blocks per
+// wavefront. dst must have room for the dword holding the last byte.
 __global__ __launch_bounds__(256) void k_compact(const uint8_t* __restrict__ slots, uint32_t slot_bytes,
                                                  const uint64_t* __restrict__ boff, uint64_t nblocks,
                                                  uint8_t* __restrict__ dst) {
-  const uint64_t b = (uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6);
+  const uint64_t b = ((uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6)) * 8 + (lane_id() >> 3);
   if (b >= nblocks) return;
   const uint64_t o = boff[b], o1 = boff[b + 1];
   const uint8_t* s = slots + b * slot_bytes;
-  const uint8_t* sn = slots + (b + 1 < nblocks ? b + 1 : b) * slot_bytes;
+  const bool has_next = b + 1 < nblocks;
+  const uint8_t* sn = slots + (has_next ? b + 1 : b) * slot_bytes;
+  const uint32_t sh = slot_shift(s), shn = slot_shift(sn);
   uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
   const uint64_t d0 = (o + 3) >> 2, d1 = (o1 + 3) >> 2;  // dwords whose first byte is in this block
-  for (uint64_t d = d0 + lane_id(); d < d1; d += 64) {
+  for (uint64_t d = d0 + (lane_id() & 7); d < d1; d += 8) {
     const uint64_t p = 4 * d;            // first byte of the dword, in [o, o1)
-    // in <= MaxBlockLength - 1 for every block the encoders produce; the
-    // clamp keeps reads inside the slot for an index built over rejected
-    // blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
-    const uint32_t in = (uint32_t)min(p - o, (uint64_t)(slot_bytes - 8));
+    // sh + (p - o) <= 3 + MaxBlockLength - 1 for every block the encoders
+    // produce; the clamp keeps reads inside the slot for an index built over
+    // rejected blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
+    const uint32_t in = (uint32_t)min(p - o + sh, (uint64_t)(slot_bytes - 4));
     uint32_t w = slot_u32(s, in);
     const uint64_t mine = o1 - p;        // bytes of the dword in this block (>= 1)
     if (mine < 4) {
       const uint32_t keep = (1u << (8 * mine)) - 1u;
-      w = (w & keep) | (b + 1 < nblocks ? (slot_u32(sn, 0) << (8 * mine)) : 0u);
+      w = (w & keep) | (has_next ? (slot_u32(sn, shn) << (8 * mine)) : 0u);
     }
     d32[d] = w;
   }
@@ -388,9 +734,10 @@ hipError_t utl_launch_encode_u32(const uint32_t* values, const uint64_t* in_off,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  uint32_t* work_out, hipStream_t stream) {
   if (nblocks == 0) return hipSuccess;
-  const uint64_t grid = (nblocks + kEncWaves - 1) / kEncWaves;
-  k_encode_u32<<<dim3((unsigned)grid), dim3(64 * kEncWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
-                                                                            slot_bytes, out_meta, work_out);
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  const uint64_t grid = (nblocks + per_wg - 1) / per_wg;
+  k_encode_u32<<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
+                                                                               slot_bytes, out_meta, work_out);
   return hipGetLastError();
 }
 
@@ -407,7 +754,7 @@ hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off,
 hipError_t utl_launch_compact(const uint8_t* slots, uint32_t slot_bytes, const uint64_t* boff, uint64_t nblocks,
                               uint8_t* dst, hipStream_t stream) {
   if (nblocks == 0) return hipSuccess;
-  k_compact<<<dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, stream>>>(slots, slot_bytes, boff, nblocks, dst);
+  k_compact<<<dim3((unsigned)((nblocks + 31) / 32)), dim3(256), 0, stream>>>(slots, slot_bytes, boff, nblocks, dst);
   return hipGetLastError();
 }
 
