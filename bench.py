@@ -80,7 +80,7 @@ def main() -> int:
     ap.add_argument("--values", type=int, default=64 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step (for profiling)")
-    ap.add_argument("--workload", default="delta64m", choices=["delta64m", "plain1m", "uneven", "u64", "serve"],
+    ap.add_argument("--workload", default="delta64m", choices=["delta64m", "plain1m", "uneven", "u64", "serve", "walk"],
                     help="delta64m is the headline; the others print their own JSON lines")
     ap.add_argument("--store-gb", type=float, default=16.0, help="serve: compressed store size")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: offered requests per second")
@@ -105,6 +105,9 @@ def main() -> int:
             print(json.dumps(runbench.plain_1m(ctx)), flush=True)
         elif args.workload == "uneven":
             print(json.dumps(runbench.uneven(ctx, nstreams=args.streams, progress=log)), flush=True)
+        elif args.workload == "walk":
+            for rec in runbench.walk_delta(ctx, args.values, progress=log):
+                print(json.dumps(rec), flush=True)
         elif args.workload == "u64":
             for rec in runbench.u64_and_get(ctx):
                 print(json.dumps(rec), flush=True)

@@ -146,6 +146,27 @@ int utl_index_build(utl_ctx *ctx, const uint32_t *d_meta, size_t nblocks, uint64
 int utl_block_lengths(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_start, const uint64_t *d_limit,
                       size_t nblocks, uint32_t *d_meta, int32_t *d_status);
 
+/* Index of foreign streams whose block starts are unknown: the reference's
+ * serial walk (block k+1 starts BlockLength(block k) bytes after block k,
+ * block_length.go:8, example_test.go:171-187) on the device, one wavefront
+ * per stream. Stream s is d_src[d_stream_off[s] : d_stream_off[s+1]]. Its
+ * walk stops at the stream's end (d_status[s] = 0), at the first BlockLength
+ * error (that error), or at a block longer than the bytes left
+ * (ErrInvalidBuffer); d_nblocks[s] counts the blocks before the stop. With
+ * d_meta NULL only counts and statuses are written (pass 1); otherwise the
+ * blocks' meta words (length | count << 24, as the encoders emit them) go to
+ * d_meta[d_blk_off[s] ..] and, when d_block_start is not NULL, their byte
+ * offsets in d_src to d_block_start[d_blk_off[s] ..], d_blk_off being the
+ * exclusive scan of the pass-1 counts (pass 2). Block k then decodes as
+ * d_src[start[k] : start[k+1]]: its exact bytes inside a stream, the rest of
+ * its stream after a stop (as UnpackUint32(buf[pos:]) reads it,
+ * example_test.go:183). utl_index_build over d_meta gives the value offsets.
+ * d_src must be 16-byte aligned; no byte past d_stream_off[nstreams] is
+ * read. */
+int utl_index_walk(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_stream_off, size_t nstreams,
+                   const uint64_t *d_blk_off, uint32_t *d_meta, uint64_t *d_block_start, uint32_t *d_nblocks,
+                   int32_t *d_status);
+
 /* Batched UnpackUint32 over an index (one launch). d_status[k] receives the
  * block's status word; d_consumed (optional, may be NULL) its consumed
  * bytes. Values of failed blocks are not written. */

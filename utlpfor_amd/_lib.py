@@ -52,6 +52,7 @@ SIGNATURES = {
     "utl_get_tune": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "utl_get_batch": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp, sz, vp, vp]),
     "utl_gather_index": (C.c_int, [vp, vp, vp, sz, vp, sz, sz, vp, vp]),
+    "utl_index_walk": (C.c_int, [vp, vp, vp, sz, vp, vp, vp, vp, vp]),
     "utl_encode_blocks": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp]),
     "utl_compact": (C.c_int, [vp, C.c_int, vp, vp, sz, vp]),
     "utl_make_jobs": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),

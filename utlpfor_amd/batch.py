@@ -109,6 +109,47 @@ def from_host_blocks(ctx: Context, stream_bytes: np.ndarray, block_len: np.ndarr
                         is64)
 
 
+def index_foreign(ctx: Context, data: DeviceArray, stream_off: np.ndarray, is64: bool = False):
+    """Indexes streams of concatenated blocks whose block starts are unknown
+    (stream s = data[stream_off[s]:stream_off[s+1]]) with the device walk.
+    Returns (DeviceStream over all walked blocks, per-stream block counts,
+    per-stream walk statuses). A stream's blocks after its stop (its first
+    BlockLength error or a truncated block) are not indexed. block_off[k] is
+    the walked start of block k, so block k decodes from its start to the
+    next walked block's start (its exact end inside a stream, the rest of the
+    stream after a stop, like UnpackUint32(buf[pos:]) in example_test.go:183);
+    the last entry is the last block's end. `data` must be 16-byte aligned."""
+    lib = _lib.load()
+    stream_off = np.ascontiguousarray(stream_off, dtype=np.uint64)
+    ns = stream_off.size - 1
+    d_off = DeviceArray.from_numpy(stream_off, ctx)
+    d_n = DeviceArray(max(ns, 1), np.uint32, ctx)
+    d_st = DeviceArray(max(ns, 1), np.int32, ctx)
+    check(lib.utl_index_walk(ctx.handle, data.ptr, d_off.ptr, ns, None, None, None, d_n.ptr, d_st.ptr),
+          "walk (count)")
+    counts = d_n.to_numpy(ns).astype(np.uint64)
+    status = d_st.to_numpy(ns)
+    blk_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    nblocks = int(blk_off[-1])
+    d_blk = DeviceArray.from_numpy(blk_off, ctx)
+    meta = DeviceArray(max(nblocks, 1), np.uint32, ctx)
+    boff = DeviceArray(nblocks + 1, np.uint64, ctx)
+    check(lib.utl_index_walk(ctx.handle, data.ptr, d_off.ptr, ns, d_blk.ptr, meta.ptr, boff.ptr, d_n.ptr,
+                             d_st.ptr), "walk (meta)")
+    scanned, ooff = build_index(ctx, meta, nblocks)
+    nbytes = 0
+    if nblocks:
+        last_start = int(boff.read(nblocks - 1, 1)[0])
+        last_len = int(meta.read(nblocks - 1, 1)[0]) & 0xFFFFFF
+        nbytes = last_start + last_len
+    boff.write(nblocks, np.array([nbytes], dtype=np.uint64))
+    nvalues = int(ooff.read(nblocks, 1)[0])
+    ctx.sync()
+    for a in (d_off, d_n, d_st, d_blk, scanned):
+        a.free()
+    return DeviceStream(ctx, data, meta, boff, ooff, nblocks, nvalues, nbytes, is64), counts, status
+
+
 def decode(ctx: Context, s: DeviceStream, out: DeviceArray | None = None, status: DeviceArray | None = None,
            consumed: DeviceArray | None = None, persistent: bool = False):
     """One launch decoding every block of the stream into `out`."""

@@ -498,6 +498,24 @@ int utl_get_tune(utl_ctx* ctx, int* waves_per_wg, int* grid_wgs, int* blocks_per
   return UTL_OK;
 }
 
+int utl_index_walk(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_stream_off, size_t nstreams,
+                   const uint64_t* d_blk_off, uint32_t* d_meta, uint64_t* d_block_start, uint32_t* d_nblocks,
+                   int32_t* d_status) {
+  if (nstreams && (!d_src || !d_stream_off || !d_nblocks || !d_status || (d_meta && !d_blk_off) ||
+                   (d_block_start && !d_meta)))
+    return UTL_ERR_ARGUMENT;
+  if (((uintptr_t)d_src & 15) != 0) {  // the walk stages 16-byte chunks
+    g_last_error = "utl_index_walk: d_src must be 16-byte aligned";
+    return UTL_ERR_ARGUMENT;
+  }
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_index_walk(d_src, d_stream_off, nstreams, d_blk_off, d_meta, d_block_start, d_nblocks, d_status,
+                                c->stream));
+  return UTL_OK;
+}
+
 int utl_gather_index(utl_ctx* ctx, const uint64_t* d_block_off, const uint64_t* d_out_off, size_t store_nblocks,
                      const uint64_t* d_req, size_t nreq, size_t nentries, uint64_t* d_gblock_off,
                      uint64_t* d_gout_off) {

@@ -81,6 +81,25 @@ class DeviceArray:
                                           out.nbytes), "d2h")
         return out
 
+    def read(self, first: int, count: int) -> np.ndarray:
+        """Elements [first, first + count) as a host array."""
+        if first < 0 or count < 0 or first + count > self.n:
+            raise IndexError(f"read [{first}, {first + count}) of {self.n}")
+        out = np.empty(count, dtype=self.dtype)
+        if out.nbytes:
+            check(self.lib.utl_memcpy_d2h(self.ctx.handle if self.ctx else None, out.ctypes.data,
+                                          self.ptr + first * self.dtype.itemsize, out.nbytes), "d2h")
+        return out
+
+    def write(self, first: int, a: np.ndarray) -> None:
+        """Copies host array `a` into elements [first, first + a.size)."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if first < 0 or first + a.size > self.n:
+            raise IndexError(f"write [{first}, {first + a.size}) of {self.n}")
+        if a.nbytes:
+            check(self.lib.utl_memcpy_h2d(self.ctx.handle if self.ctx else None,
+                                          self.ptr + first * self.dtype.itemsize, a.ctypes.data, a.nbytes), "h2d")
+
     def fill_bytes(self, value: int) -> None:
         check(self.lib.utl_memset_d(self.ctx.handle if self.ctx else None, self.ptr, value, self.nbytes), "fill")
 

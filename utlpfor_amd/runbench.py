@@ -38,6 +38,51 @@ def _decode_record(name, s, ms, nvalues, itemsize, extra=None):
     return rec
 
 
+def walk(ctx: Context, s: batch.DeviceStream, stream_off: np.ndarray, reps: int = 3) -> dict:
+    """Times the foreign-stream index walk (count pass, then meta + starts
+    pass) over the stream bytes of `s`, cut at `stream_off`, and checks the
+    walked index against the encoder's own (meta words and block offsets)."""
+    lib = _lib.load()
+    ns = stream_off.size - 1
+    d_off = DeviceArray.from_numpy(np.ascontiguousarray(stream_off, dtype=np.uint64), ctx)
+    d_n = DeviceArray(max(ns, 1), np.uint32, ctx)
+    d_st = DeviceArray(max(ns, 1), np.int32, ctx)
+    count_ms = _time(ctx, lambda: check(lib.utl_index_walk(ctx.handle, s.data.ptr, d_off.ptr, ns, None, None, None,
+                                                           d_n.ptr, d_st.ptr), "walk (count)"), reps)
+    counts = d_n.to_numpy(ns).astype(np.uint64)
+    blk_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    nb = int(blk_off[-1])
+    d_blk = DeviceArray.from_numpy(blk_off, ctx)
+    meta = DeviceArray(max(nb, 1), np.uint32, ctx)
+    start = DeviceArray(max(nb, 1), np.uint64, ctx)
+    meta_ms = _time(ctx, lambda: check(lib.utl_index_walk(ctx.handle, s.data.ptr, d_off.ptr, ns, d_blk.ptr, meta.ptr,
+                                                          start.ptr, d_n.ptr, d_st.ptr), "walk (meta)"), reps)
+    ok = nb == s.nblocks and bool(np.all(d_st.to_numpy(ns) == 0))
+    if ok:
+        ok = bool(np.array_equal(meta.to_numpy(nb), s.meta.to_numpy(nb)) and
+                  np.array_equal(start.to_numpy(nb), s.block_off.to_numpy(nb)))
+    for a in (d_off, d_n, d_st, d_blk, meta, start):
+        a.free()
+    return {"streams": ns, "blocks": nb, "count_pass_ms": round(count_ms, 4), "meta_pass_ms": round(meta_ms, 4),
+            "walked_GBps": round(s.nbytes / (meta_ms * 1e-3) / 1e9, 1), "index_equal_encoder": ok}
+
+
+def walk_delta(ctx: Context, n: int = 64 << 20, progress=None):
+    """The foreign-stream walk over the delta-postings stream of n values
+    (flag Delta), read as one stream and cut into streams of 1024 blocks."""
+    say = progress or (lambda m: None)
+    v = workloads.delta_postings(n)
+    s = batch.encode(ctx, v, flag=1)
+    boff = s.block_off.to_numpy()
+    for per in (s.nblocks, 1024, 128):
+        cuts = np.unique(np.append(np.arange(0, s.nblocks, per), s.nblocks))
+        say(f"walk: {cuts.size - 1} streams of up to {per} blocks")
+        rec = {"workload": f"foreign-stream walk, delta postings {n} u32, streams of up to {per} blocks"}
+        rec.update(walk(ctx, s, boff[cuts]))
+        yield rec
+    s.free()
+
+
 def plain_1m(ctx: Context, reps: int = 20) -> dict:
     v = workloads.plain_1m()
     s = batch.encode(ctx, v, flag=0)
@@ -75,11 +120,13 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
         check(_lib.load().utl_memcpy_d2h(ctx.handle, got.ctypes.data, out.ptr + 4 * a, 4 * n), "d2h")
         ok &= bool(np.array_equal(got, v[a:a + n]))
     graph.close()
+    say("foreign-stream index walk")
+    wrec = walk(ctx, s, s.block_off.to_numpy()[batch.plan_blocks(lens).astype(np.int64)])
     rec = _decode_record(f"{nstreams} uneven streams, lengths logU[1,200000], mixed flags", s, ms, v.size, 4,
                          {"bit_exact": ok, "form": "persistent decode + index scan in one hipGraph launch",
                           "flat_decode_ms": round(flat_ms, 5),
                           "tune": {k: tune[k] for k in ("grid_wgs", "tiles_per_grab", "ms", "flat_ms")},
-                          "streams": nstreams, "gen_seconds": round(gen_s, 1)})
+                          "streams": nstreams, "gen_seconds": round(gen_s, 1), "walk": wrec})
     s.free()
     return rec
 
