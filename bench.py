@@ -52,6 +52,23 @@ def cpu_baseline(stream_bytes, boff, ooff, nvalues, target_s: float = 12.0):
                       f"reference scalar path on {threads} threads ({el:.1f} s wall)"}
 
 
+def walk_cpu_baseline(stream_bytes, target_s: float = 3.0):
+    """The oracle's serial BlockLength walk (example_test.go:171-187) over
+    the same stream, one thread (the walk is serial)."""
+    from oracle import oracle
+    oracle.walk_stream(stream_bytes)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        st, _, _, _ = oracle.walk_stream(stream_bytes)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    return {"value": round(el / reps * 1e3, 3), "unit": "ms per walk", "cores": 1, "kind": "port",
+            "sample": f"the whole {stream_bytes.size}-byte stream ({st.size} blocks) walked {reps}x by the C "
+                      f"restatement of the reference's BlockLength walk on one thread"}
+
+
 def per_call_form(ctx, stream_bytes, boff, nblocks: int, calls: int = 2000):
     """One C-ABI UnpackUint32 call per block (host bytes in, host values out)."""
     from utlpfor_amd import codec
@@ -106,7 +123,11 @@ def main() -> int:
         elif args.workload == "uneven":
             print(json.dumps(runbench.uneven(ctx, nstreams=args.streams, progress=log)), flush=True)
         elif args.workload == "walk":
-            for rec in runbench.walk_delta(ctx, args.values, progress=log):
+            for i, rec in enumerate(runbench.walk_delta(ctx, args.values, progress=log)):
+                if i == 0 and not args.no_cpu:
+                    log("CPU baseline: the oracle's serial walk of the same stream")
+                    rec["cpu_baseline"] = walk_cpu_baseline(rec.pop("stream_bytes"))
+                rec.pop("stream_bytes", None)
                 print(json.dumps(rec), flush=True)
         elif args.workload == "u64":
             for rec in runbench.u64_and_get(ctx):

@@ -161,8 +161,10 @@ int utl_block_lengths(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_bloc
  * d_src[start[k] : start[k+1]]: its exact bytes inside a stream, the rest of
  * its stream after a stop (as UnpackUint32(buf[pos:]) reads it,
  * example_test.go:183). utl_index_build over d_meta gives the value offsets.
- * d_src must be 16-byte aligned; no byte past d_stream_off[nstreams] is
- * read. */
+ * d_src must be 16-byte aligned and d_stream_off non-decreasing; no byte
+ * of a stream past its end is read. The call reads d_stream_off[0] and
+ * d_stream_off[nstreams] back to size its scratch (one synchronization of
+ * the context's stream); the rest is asynchronous. */
 int utl_index_walk(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_stream_off, size_t nstreams,
                    const uint64_t *d_blk_off, uint32_t *d_meta, uint64_t *d_block_start, uint32_t *d_nblocks,
                    int32_t *d_status);

@@ -67,6 +67,8 @@ def lib():
         for f in (L.orc_unpack_blocks_u32, L.orc_unpack_blocks_u64):
             f.argtypes = [_u8p, _u64p, _u64p, C.c_size_t, C.c_void_p, _i32p, C.c_int]
             f.restype = None
+        L.orc_walk_stream.argtypes = [_u8p, C.c_size_t, _u64p, _u32p, _u32p, C.c_size_t, _i32p]
+        L.orc_walk_stream.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -189,3 +191,18 @@ def unpack_blocks(src: np.ndarray, off: np.ndarray, out_off: np.ndarray, n_value
     f(src.ctypes.data_as(_u8p), off.ctypes.data_as(_u64p), out_off.ctypes.data_as(_u64p), nb,
       out.ctypes.data, status.ctypes.data_as(_i32p), nthreads)
     return out, status
+
+
+def walk_stream(src: np.ndarray):
+    """Serial BlockLength walk of one stream (example_test.go:171-187).
+    Returns (starts uint64, lengths uint32, counts uint32, status)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    pad = np.concatenate([src, np.zeros(1, np.uint8)])  # keeps empty streams addressable
+    cap = src.size // 4 + 1  # a block is at least 4 bytes
+    starts = np.zeros(cap, np.uint64)
+    lens = np.zeros(cap, np.uint32)
+    counts = np.zeros(cap, np.uint32)
+    st = C.c_int32()
+    n = lib().orc_walk_stream(pad.ctypes.data_as(_u8p), src.size, starts.ctypes.data_as(_u64p),
+                              lens.ctypes.data_as(_u32p), counts.ctypes.data_as(_u32p), cap, C.byref(st))
+    return starts[:n], lens[:n], counts[:n], st.value

@@ -1194,3 +1194,27 @@ void orc_unpack_blocks_u64(const uint8_t *src, const uint64_t *off, const uint64
   p.status = status;
   run_parallel(&p, nblocks, nthreads);
 }
+
+int64_t orc_walk_stream(const uint8_t *src, size_t len, uint64_t *starts, uint32_t *lens, uint32_t *counts,
+                        size_t cap, int32_t *status) {
+  size_t pos = 0;
+  int64_t n = 0;
+  *status = ORC_OK;
+  while (pos < len) {
+    int64_t bl = 0;
+    int err = orc_block_length(src + pos, len - pos, &bl);
+    if (err == ORC_OK && (uint64_t)bl > len - pos) err = ORC_ERR_INVALID_BUFFER;
+    if (err != ORC_OK) {
+      *status = err;
+      break;
+    }
+    if ((size_t)n < cap) {
+      starts[n] = pos;
+      lens[n] = (uint32_t)bl;
+      counts[n] = src[pos];
+    }
+    ++n;
+    pos += (size_t)bl;
+  }
+  return n;
+}

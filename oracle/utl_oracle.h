@@ -103,6 +103,15 @@ void orc_unpack_blocks_u64(const uint8_t *src, const uint64_t *off,
                            const uint64_t *out_off, size_t nblocks,
                            uint64_t *out, int32_t *status, int nthreads);
 
+/* The caller's serial walk of one stream (example_test.go:171-187): block
+ * k+1 starts BlockLength(block k) bytes after block k. Stops at the end of
+ * src (*status = 0), at the first BlockLength error (*status = that error)
+ * or at a block longer than the bytes left (*status = ErrInvalidBuffer).
+ * Writes the first min(blocks, cap) blocks' start offsets, byte lengths and
+ * value counts (header byte 0); returns the number of blocks walked. */
+int64_t orc_walk_stream(const uint8_t *src, size_t len, uint64_t *starts, uint32_t *lens, uint32_t *counts,
+                        size_t cap, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

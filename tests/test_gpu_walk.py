@@ -31,20 +31,8 @@ def ctx():
 
 def host_walk(orc, buf: bytes):
     """Serial walk of one stream: (starts, lengths, counts, status)."""
-    starts, lens, counts = [], [], []
-    pos, st = 0, 0
-    while pos < len(buf):
-        bl, err = orc.BlockLength(buf[pos:])
-        if err == 0 and bl > len(buf) - pos:
-            err = 1
-        if err:
-            st = err
-            break
-        starts.append(pos)
-        lens.append(bl)
-        counts.append(buf[pos])
-        pos += bl
-    return starts, lens, counts, st
+    st, ln, ct, status = orc.walk_stream(np.frombuffer(buf, np.uint8))
+    return [int(x) for x in st], [int(x) for x in ln], [int(x) for x in ct], status
 
 
 def run_walk(ctx, orc, streams, is64):
@@ -157,3 +145,63 @@ def test_walk_empty_and_tiny(ctx, orc):
     streams = [b"", b"\x01", b"", orc.PackUint32(0, np.array([7], np.uint32))[0], b"\x00\x00\x00"]
     counts, status = run_walk(ctx, orc, streams, False)
     assert list(counts) == [0, 0, 0, 1, 0]
+
+
+def _starts(orc, b):
+    return host_walk(orc, b)[0]
+
+
+def test_walk_long_streams_chunked(ctx, orc):
+    """Streams of 32 KiB and more take the chunk-parallel walk (entry votes,
+    verification, re-walks of chunks whose vote was wrong). Against the
+    host walk: clean long streams between short ones, a 20000-byte zero run
+    between two blocks (5000 valid 4-byte count-0 blocks), a block whose
+    svbLen field jumps over several chunks, a reserved header bit and a
+    truncation in the middle of long streams."""
+    rng = np.random.default_rng(24)
+    outl = rng.integers(0, 1 << 12, 300000, dtype=np.uint64)
+    k = rng.choice(outl.size, 9000, replace=False)
+    outl[k] = rng.integers(1 << 20, 1 << 32, k.size, dtype=np.uint64)
+    longs = [orc.pack_stream(1, np.cumsum(rng.integers(0, 64, 400000)).astype(np.uint32))[0].tobytes(),
+             orc.pack_stream(0, outl.astype(np.uint32))[0].tobytes(),
+             orc.pack_stream(2, rng.integers(0, 1 << 32, 100000, dtype=np.uint64).astype(np.uint32))[0].tobytes(),
+             orc.pack_stream(4, rng.integers(0, 1 << 9, 250000, dtype=np.uint64).astype(np.uint32))[0].tobytes()]
+    # zero run between two blocks in the middle of stream 0
+    st0 = _starts(orc, longs[0])
+    cut = st0[len(st0) // 2]
+    zero_run = longs[0][:cut] + bytes(20000) + longs[0][cut:]
+    # stream 1: a block with exceptions gets svbLen 60000
+    a = bytearray(longs[1])
+    st1 = _starts(orc, longs[1])
+    jb = next(p for p in st1[len(st1) // 3:] if a[p + 3] != 0)
+    a[jb + 4:jb + 6] = (60000).to_bytes(2, "little")
+    jump = bytes(a)
+    # stream 2: reserved bit in a block two thirds in
+    a = bytearray(longs[2])
+    st2 = _starts(orc, longs[2])
+    a[st2[2 * len(st2) // 3] + 2] |= 0x04
+    reserved = bytes(a)
+    truncated = longs[3][:-7]
+    shorts, _ = make_streams(orc, rng, 30, 3000, False)
+    streams = []
+    for i, b in enumerate([longs[0], zero_run, longs[1], jump, longs[2], reserved, longs[3], truncated]):
+        streams += [shorts[i], b, shorts[i + 8]]
+    streams += shorts[16:]
+    say(f"walk long: {len(streams)} streams, {sum(map(len, streams))} bytes")
+    counts, status = run_walk(ctx, orc, streams, False)
+    big = [i for i, b in enumerate(streams) if len(b) >= 32768]
+    assert len(big) >= 8
+    assert status[streams.index(reserved)] == 4
+    assert status[streams.index(truncated)] == 1
+    assert counts[streams.index(zero_run)] == counts[streams.index(longs[0])] + 5000
+
+
+def test_walk_long_streams_u64(ctx, orc):
+    rng = np.random.default_rng(25)
+    ts = (int(1.7e18) + np.cumsum(rng.geometric(1 / 1000, 200000))).astype(np.uint64)
+    uni = rng.integers(0, 1 << 64, 60000, dtype=np.uint64)
+    streams = [orc.pack_stream(0, ts)[0].tobytes(), orc.pack_stream(1, ts)[0].tobytes(),
+               orc.pack_stream(0, uni)[0].tobytes()]
+    say(f"walk long u64: {len(streams)} streams, {sum(map(len, streams))} bytes")
+    counts, status = run_walk(ctx, orc, streams, True)
+    assert np.all(status == 0)

@@ -9,6 +9,8 @@ operand) fails at least one of them:
 * Example sizes            example_test.go:10-340
 * MaxBlockLength           max_block_length_test.go:12-44, example_test.go:302-330
 * BlockLength table/errs   block_length_test.go:37-59, :127-176
+* stream walk              example_test.go:171-187 (BlockLength skip), packed
+                           block lengths, truncation and header stops
 * overflow position        delta_test.go:229-244, simd_delta_overflow_test.go:177-215
 * FOR bases and widths     for_test.go:110-175, :457-478
 * FOR64 layout             uint64_test.go:1143-1214
@@ -439,3 +441,38 @@ def test_stream_helpers_match_per_block(orc):
     for k in range(bl.size):
         blk, _, _ = orc.PackUint32(orc.Append, values[128 * k:128 * k + 128])
         assert blk == stream[int(off[k]):int(off[k + 1])].tobytes()
+
+
+def test_walk_stream(orc):
+    """The serial stream walk: the example's two-block skip
+    (example_test.go:171-187), block starts equal to the scanned packed
+    lengths, a per-block BlockLength loop, and its three stops."""
+    P = lambda f, v: orc.PackUint32(f, v)[0]
+    buf = np.frombuffer(P(orc.Append, [1, 2, 3]) + P(orc.Append, [100, 200, 300]), np.uint8)
+    st, ln, ct, status = orc.walk_stream(buf)
+    assert list(st) == [0, 13] and list(ln) == [13, buf.size - 13] and list(ct) == [3, 3] and status == 0
+    rng = np.random.default_rng(3)
+    for flag in (0, orc.Delta, orc.NoPatch, orc.Delta | orc.NoFOR):
+        v = rng.integers(0, 1 << int(rng.integers(1, 33)), 5000, dtype=np.uint64).astype(np.uint32)
+        stream, bl, bc = orc.pack_stream(flag, v)
+        st, ln, ct, status = orc.walk_stream(stream)
+        assert status == 0 and np.array_equal(ln, bl) and np.array_equal(ct, bc)
+        assert np.array_equal(st, np.concatenate([[0], np.cumsum(bl.astype(np.uint64))[:-1]]))
+        # the same walk, one BlockLength call per block
+        b = stream.tobytes()
+        pos, k = 0, 0
+        while pos < len(b):
+            L, err = orc.BlockLength(b[pos:])
+            assert err == 0 and st[k] == pos and ln[k] == L
+            pos, k = pos + L, k + 1
+        assert k == st.size
+        # stops: a truncated last block, a reserved header bit, a short tail
+        st2, _, _, s2 = orc.walk_stream(stream[:-1])
+        assert s2 == orc.ERR_INVALID_BUFFER and st2.size == st.size - 1
+        bad = stream.copy()
+        bad[int(st[7]) + 2] |= 0x04
+        st3, _, _, s3 = orc.walk_stream(bad)
+        assert s3 == orc.ERR_INVALID_FLAGS and st3.size == 7
+        st4, _, _, s4 = orc.walk_stream(np.concatenate([stream, np.zeros(3, np.uint8)]))
+        assert s4 == orc.ERR_INVALID_BUFFER and st4.size == st.size
+    assert orc.walk_stream(np.zeros(0, np.uint8))[3] == 0

@@ -511,8 +511,17 @@ int utl_index_walk(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_stream_
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_index_walk(d_src, d_stream_off, nstreams, d_blk_off, d_meta, d_block_start, d_nblocks, d_status,
-                                c->stream));
+  if (nstreams == 0) return UTL_OK;
+  // the first and last stream offsets size the chunk records
+  HIP_TRY(hipMemcpyAsync(c->h_stage, d_stream_off, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->h_stage + 8, d_stream_off + nstreams, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  uint64_t ends[2];
+  memcpy(ends, c->h_stage, 16);
+  const uint64_t total = ends[1] > ends[0] ? ends[1] - ends[0] : 0;
+  if ((err = ensure_tmp(c, utl_walk_tmp_bytes(nstreams, total)))) return err;
+  HIP_TRY(utl_launch_index_walk(d_src, d_stream_off, nstreams, total, d_blk_off, d_meta, d_block_start, d_nblocks,
+                                d_status, c->d_tmp, c->stream));
   return UTL_OK;
 }
 

@@ -133,84 +133,6 @@ __global__ void k_block_lengths(const uint8_t* __restrict__ src, const uint64_t*
   meta[b] = st == UTL_OK ? ((uint32_t)(bl & 0xFFFFFF) | (count << 24)) : 0u;
 }
 
-// Walk of foreign streams whose block starts are unknown (the reference's
-// serial walk: block k+1 starts BlockLength(block k) bytes after block k,
-// block_length.go:8, example_test.go:171-187). One wavefront per stream: it
-// stages a 4 KiB window of the stream into LDS with coalesced 16-byte loads
-// and walks the headers from LDS (BlockLength reads at most the first 16
-// bytes of a block), restaging when a block's first 16 bytes leave the
-// window. The walk of stream s stops at its end (status 0), at the first
-// BlockLength error (that error), or at a block longer than the bytes left
-// (ErrInvalidBuffer: the block is not present). nblocks[s] counts the
-// blocks before the stop; with meta set, their meta words (length | count
-// << 24) go to meta[blk_off[s] ..] and, with start set, their byte offsets
-// in src to start[blk_off[s] ..].
-constexpr int kWalkWin = 4096;
-
-__global__ __launch_bounds__(256) void k_index_walk(const uint8_t* __restrict__ src,
-                                                    const uint64_t* __restrict__ stream_off, uint64_t nstreams,
-                                                    const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
-                                                    uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
-                                                    int32_t* __restrict__ status) {
-  __shared__ uint4 win[4][kWalkWin / 16 + 2];  // (+ one dword read past a block's 16 bytes)
-  const uint32_t wave = uniform(threadIdx.x >> 6);
-  uint4* W = win[wave];
-  const uint32_t* W32 = reinterpret_cast<const uint32_t*>(W);
-  const uint64_t step = (uint64_t)gridDim.x * 4;
-  for (uint64_t s = (uint64_t)blockIdx.x * 4 + wave; s < nstreams; s += step) {
-    const uint64_t end = stream_off[s + 1];
-    uint64_t pos = stream_off[s];
-    uint64_t out = meta ? blk_off[s] : 0;
-    uint32_t n = 0;
-    int32_t st = UTL_OK;
-    uint64_t wbase = 1, wend = 0;  // empty window
-    while (pos < end) {
-      if (pos < wbase || (pos + 16 > wend && wend < end)) {
-        // stage the 16-byte chunks holding [pos, min(end, pos + window))
-        wbase = pos & ~15ull;
-        wend = min(end, wbase + kWalkWin);
-        const int nch = (int)((wend - wbase + 15) >> 4);
-        const uint4* g = reinterpret_cast<const uint4*>(src + wbase);
-        for (int c = lane_id(); c < nch; c += 64) {
-          const uint64_t cb = wbase + 16ull * (uint64_t)c;
-          if (cb + 16 <= end) {
-            W[c] = g[c];
-          } else {  // the stream's last, partial chunk: read no byte past its end
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
-            for (int k = 0; k < (int)(end - cb); ++k) w[k >> 2] |= (uint32_t)src[cb + k] << (8 * (k & 3));
-            W[c] = make_uint4(w[0], w[1], w[2], w[3]);
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      const int at = (int)(pos - wbase);
-      const int64_t left = (int64_t)(end - pos);
-      int64_t bl = 0;
-      // little-endian dword at window byte at + off (off <= 14)
-      auto rd32 = [W32, at](int off) {
-        const int i = at + off;
-        return __builtin_amdgcn_alignbyte(W32[(i >> 2) + 1], W32[i >> 2], i & 3);
-      };
-      st = block_length_impl(
-          left, rd32, [&rd32](int off) { return rd32(off) & 0xFFFFu; }, &bl);
-      if (st == UTL_OK && bl > left) st = UTL_ERR_INVALID_BUFFER;
-      if (st != UTL_OK) break;
-      if (meta && lane_id() == 0) {
-        meta[out + n] = (uint32_t)(bl & 0xFFFFFF) | ((rd32(0) & 0xFFu) << 24);
-        if (start) start[out + n] = pos;
-      }
-      ++n;
-      pos += (uint64_t)bl;
-    }
-    if (lane_id() == 0) {
-      nblocks[s] = n;
-      status[s] = st;
-    }
-  }
-}
-
 __global__ void k_zero_first(uint64_t* a, uint64_t* b) {
   a[0] = 0;
   b[0] = 0;
@@ -254,17 +176,6 @@ hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, c
                                     uint32_t* meta, int32_t* status, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   k_block_lengths<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream>>>(src, start, limit, n, meta, status);
-  return hipGetLastError();
-}
-
-hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
-                                 const uint64_t* blk_off, uint32_t* meta, uint64_t* start, uint32_t* nblocks,
-                                 int32_t* status, hipStream_t stream) {
-  if (nstreams == 0) return hipSuccess;
-  const uint64_t wgs = (nstreams + 3) / 4;
-  const unsigned grid = (unsigned)(wgs < 8192 ? wgs : 8192);
-  k_index_walk<<<dim3(grid), dim3(256), 0, stream>>>(src, stream_off, nstreams, blk_off, meta, start, nblocks,
-                                                     status);
   return hipGetLastError();
 }
 

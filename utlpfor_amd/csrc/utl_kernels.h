@@ -27,10 +27,13 @@ hipError_t utl_launch_index(const uint32_t* meta, uint64_t nblocks, uint64_t* bo
 hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, const uint64_t* limit, uint64_t nblocks,
                                     uint32_t* meta, int32_t* status, hipStream_t stream);
 
-// Walk of foreign streams (block starts unknown): counts, then meta words.
+// Walk of foreign streams (block starts unknown): counts, then meta words
+// (utl_walk.hip). total_bytes = stream_off[nstreams] - stream_off[0] sizes
+// the scratch tmp (utl_walk_tmp_bytes bytes).
+size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes);
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
-                                 const uint64_t* blk_off, uint32_t* meta, uint64_t* start, uint32_t* nblocks,
-                                 int32_t* status, hipStream_t stream);
+                                 uint64_t total_bytes, const uint64_t* blk_off, uint32_t* meta, uint64_t* start,
+                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream);
 
 hipError_t utl_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t stream);
 hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t stream);

@@ -1,0 +1,596 @@
+// utl_walk.hip -- index of foreign streams whose block starts are unknown.
+//
+// The reference finds blocks by a serial walk: block k+1 starts
+// BlockLength(block k) bytes after block k (block_length.go:8,
+// example_test.go:171-187). The walk of stream s stops at the stream's end
+// (status 0), at the first BlockLength error (that error) or at a block
+// longer than the bytes left (ErrInvalidBuffer). The kernels here give the
+// same blocks and statuses.
+//
+// Short streams (< kLong bytes): one wavefront per stream walks it through
+// 16 KiB LDS windows (k_walk_serial).
+//
+// Long streams are cut into kChunk-byte chunks and walked chunk-parallel:
+//   k_walk_vote    per chunk, a guess of its entry (the first block start at
+//                  or after the chunk start): every candidate position of a
+//                  kWarm-byte warm-up region before the chunk is followed by
+//                  pointer jumping in LDS to the point where it crosses into
+//                  the chunk, and the crossing most candidates reach wins
+//                  (walks from wrong positions die at a failed header or
+//                  merge into the stream's own chain);
+//   k_walk_chunks  one wavefront per chunk walks it from the guess: block
+//                  count, exit (first position at or past the chunk end) and
+//                  stop status;
+//   k_walk_verify  one wavefront per stream checks, 64 chunks at a time,
+//                  that each chunk's guess equals the previous chunk's exit;
+//                  the first chunk that does not is walked again from that
+//                  exit. The result is exact whatever the guesses were: a
+//                  chunk counts only once its entry is the true one, and the
+//                  first chunk's entry is the stream start.
+//   k_walk_emit    (second pass) one wavefront per live chunk walks it again
+//                  and writes its blocks' meta words and starts.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "utl_device.h"
+#include "utl_format.h"
+#include "utl_kernels.h"
+
+namespace utl {
+namespace {
+
+constexpr int kChunk = 16320;            // bytes of a long stream per chunk (window: 16 KiB)
+constexpr int kWarm = 2048;              // warm-up bytes before a chunk
+constexpr int kLand = 2048;              // entry candidates (the warm-up's first bytes); above every block
+                                         // length an encoder emits (2038)
+constexpr uint64_t kLong = 2 * kChunk;   // streams at least this long are walked chunk-parallel
+constexpr int kWin16 = 256;              // LDS window of a walking wavefront (4 KiB), in 16-byte units
+constexpr int kStep = kWin16 * 16 - 64;  // bytes walked per window (the rest: header lookahead, alignment)
+constexpr int kWarm16 = (kWarm + 64) / 16;  // LDS window of one vote
+constexpr uint64_t kNone = ~0ull;
+constexpr uint16_t kDead = 0xFFFF;
+constexpr uint32_t kStLand = 0, kStEnd = 1, kStStop = 0x100;  // chunk exit kinds (kStStop | error)
+
+struct WalkRec {
+  uint64_t e;     // entry (stream-relative), kNone if the vote found none
+  uint64_t x;     // exit: first chain position at or past the chunk end (kStLand)
+  uint32_t n;     // blocks starting in [e, x)
+  uint32_t st;    // kStLand, kStEnd or kStStop | error
+  uint32_t base;  // blocks of the stream before this chunk (written by k_walk_verify)
+  uint32_t pad;
+};
+static_assert(sizeof(WalkRec) == 32, "WalkRec layout");
+
+// BlockLength (block_length.go:8) of the block whose first bytes are
+// h = bytes 0..3, d1 = bytes 4..7, d2 = bytes 8..11 (little-endian), with
+// `left` bytes from the block start to the end of its stream. Branch-free;
+// the error precedence is the reference's: short header, count > 128,
+// bit width > 32, invalid flags, short svbLen field, short block2Len field.
+__device__ __forceinline__ int step_len(uint32_t h, uint32_t d1, uint32_t d2, int64_t left, uint32_t* bl) {
+  const uint32_t t = (h >> 13) & 3u, fw = (h >> 15) & 3u, exc = h >> 24, bwf = (h >> 8) & 31u;
+  const bool t3 = t == 3u;
+  const bool combine = t3 && (h & kHCombine) != 0u;
+  const uint32_t bad = h & (kHReserved | kHBlock256 | (t3 ? 0u : kHCombine));
+  const uint32_t forb = (t3 && !combine && fw == 3u) ? (uint32_t)kFor64 : ((1u << fw) >> 1);
+  const bool he = exc != 0u;
+  const uint32_t b2off = (uint32_t)kHdr + (he ? (uint32_t)kSvbLen : 0u) + forb;  // block2Len (combine: <= 10)
+  const uint32_t poff = b2off + (combine ? (uint32_t)kB2Len : 0u);
+  const uint32_t idx = exc <= (uint32_t)kExcBitmap ? exc : (uint32_t)kExcBitmap;
+  uint32_t b = poff + (bwf << 6) + (he ? idx + (d1 & 0xFFFFu) : 0u);
+  const uint64_t q = ((uint64_t)d2 << 32) | d1;  // bytes 4..11
+  const uint32_t sh = combine ? 8u * (b2off - 4u) : 0u;
+  b += combine ? ((uint32_t)(q >> sh) & 0xFFFFu) : 0u;
+  *bl = b;
+  int err = UTL_OK;
+  if (combine && left < (int64_t)poff) err = UTL_ERR_INVALID_BUFFER;
+  if (he && left < kHdr + kSvbLen) err = UTL_ERR_INVALID_BUFFER;
+  if (bad) err = UTL_ERR_INVALID_FLAGS;
+  if (bwf > 8u) err = UTL_ERR_INVALID_BUFFER;
+  if ((h & 0xFFu) > (uint32_t)kBlock) err = UTL_ERR_INVALID_BLOCK_LENGTH;
+  if (left < kHdr) err = UTL_ERR_INVALID_BUFFER;
+  return err;
+}
+
+// Stages the bytes [lo, hi) of src into W (CAP16 16-byte units) from the
+// 16-byte boundary at or below lo; no byte at or past hi is read (those
+// slots read as zero). Returns the window's absolute base. Threads
+// [0, NTHR) of the caller take part, each with all its loads in flight at
+// once; the caller synchronizes afterwards.
+template <int CAP16, int NTHR>
+__device__ __forceinline__ uint64_t stage(uint4* W, const uint8_t* __restrict__ src, uint64_t lo, uint64_t hi,
+                                          int tid) {
+  constexpr int U = (CAP16 + NTHR - 1) / NTHR;
+  const uint64_t base = lo & ~15ull;
+  const uint64_t span = hi > base ? hi - base : 0;
+  const int nfull = (int)min(span >> 4, (uint64_t)CAP16);  // whole 16-byte units below hi
+  uint4 v[U];
+  if (nfull > 0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = min(u * NTHR + tid, nfull - 1);
+      v[u] = *reinterpret_cast<const uint4*>(src + base + 16ull * (uint64_t)c);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int c = u * NTHR + tid;
+    if (c < CAP16 && c != nfull) W[c] = c < nfull ? v[u] : make_uint4(0u, 0u, 0u, 0u);
+  }
+  if (tid == 0 && nfull < CAP16) {  // the unit holding hi: its bytes below hi, zeros after
+    const uint64_t cb = base + 16ull * (uint64_t)nfull;
+    const int nb = hi > cb ? (int)(hi - cb) : 0;  // < 16
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    for (int k = 0; k < nb; ++k) w[k >> 2] |= (uint32_t)src[cb + k] << (8 * (k & 3));
+    W[nfull] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  return base;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bytes 0..11 from window byte i as three dwords (one batch of LDS reads).
+__device__ __forceinline__ void win96(const uint32_t* W32, int i, uint32_t& h, uint32_t& d1, uint32_t& d2) {
+  const int w = i >> 2;
+  const uint32_t s = (uint32_t)(i & 3);
+  const uint32_t a0 = W32[w], a1 = W32[w + 1], a2 = W32[w + 2], a3 = W32[w + 3];
+  h = __builtin_amdgcn_alignbyte(a1, a0, s);
+  d1 = __builtin_amdgcn_alignbyte(a2, a1, s);
+  d2 = __builtin_amdgcn_alignbyte(a3, a2, s);
+}
+
+struct WalkRes {
+  uint64_t x;
+  uint32_t n, st;
+};
+
+// BlockLength of a block at least 16 bytes from its stream's end, where
+// none of the length checks can fail: the length, and whether the header
+// passes the count, bit-width and flag checks.
+__device__ __forceinline__ uint32_t fast_len(uint32_t h, uint32_t d1, uint32_t d2, bool* ok) {
+  const uint32_t t = (h >> 13) & 3u, fw = (h >> 15) & 3u, exc = h >> 24, bwf = (h >> 8) & 31u;
+  const bool t3 = t == 3u;
+  const bool combine = t3 && (h & kHCombine) != 0u;
+  const uint32_t bad = h & (kHReserved | kHBlock256 | (t3 ? 0u : kHCombine));
+  const uint32_t forb = (t3 && !combine && fw == 3u) ? (uint32_t)kFor64 : ((1u << fw) >> 1);
+  const bool he = exc != 0u;
+  const uint32_t b2off = (uint32_t)kHdr + (he ? (uint32_t)kSvbLen : 0u) + forb;
+  uint32_t b = b2off + (bwf << 6) + (he ? min(exc, (uint32_t)kExcBitmap) + (d1 & 0xFFFFu) : 0u);
+  if (combine) b += (uint32_t)kB2Len + ((uint32_t)((((uint64_t)d2 << 32) | d1) >> (8u * (b2off - 4u))) & 0xFFFFu);
+  *ok = (h & 0xFFu) <= (uint32_t)kBlock && bwf <= 8u && bad == 0u;
+  return b;
+}
+
+// Walks a stream (absolute start so, length Ls) from position pos while
+// pos < cend (cend - pos <= kStep). The window W32 (absolute base wabs)
+// holds the stream bytes [pos, cend + 16) wherever they exist. With meta
+// set, lane 0 writes block i's meta word to meta[out + i] and its absolute
+// start to start[out + i]. Offsets within the window are 32-bit.
+__device__ WalkRes walk_range(const uint32_t* W32, uint64_t wabs, uint64_t so, uint64_t Ls, uint64_t pos,
+                              uint64_t cend, uint32_t* __restrict__ meta, uint64_t* __restrict__ start,
+                              uint64_t out) {
+  // bytes from pos to the stream end, capped: no block is 2^31 bytes long,
+  // so a capped value never ends the stream or fails a length check
+  const uint32_t lim = (uint32_t)min(Ls - pos, (uint64_t)0x7FFFFFFF);
+  const uint32_t wlen = (uint32_t)(cend - pos);
+  const int i0 = (int)(so + pos - wabs);
+  uint32_t n = 0, st = kStLand, off = 0;
+  while (off < wlen) {
+    uint32_t h, d1, d2;
+    win96(W32, i0 + (int)off, h, d1, d2);
+    const uint32_t left = lim - off;
+    bool ok;
+    uint32_t bl = fast_len(h, d1, d2, &ok);
+    if (!ok || left < 16u) {  // the reference's checks in order
+      const int err = step_len(h, d1, d2, (int64_t)left, &bl);
+      if (err != UTL_OK) {
+        st = kStStop | (uint32_t)err;
+        break;
+      }
+    }
+    if (bl > left) {
+      st = kStStop | UTL_ERR_INVALID_BUFFER;
+      break;
+    }
+    if (meta && lane_id() == 0) {
+      meta[out + n] = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
+      if (start) start[out + n] = so + pos + off;
+    }
+    ++n;
+    off += bl;
+    if (off == lim) {
+      st = kStEnd;
+      break;
+    }
+  }
+  return {pos + off, n, st};
+}
+
+// Walks chunk k of a long stream from pos (a wave-collective call).
+// Walks from pos while pos < end through kStep-byte LDS windows (a
+// wave-collective call).
+__device__ WalkRes walk_span(uint4* W, const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos,
+                             uint64_t end, uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
+  uint32_t n = 0;
+  while (pos < end) {
+    const uint64_t wend = min(pos + (uint64_t)kStep, end);
+    const uint64_t wabs = stage<kWin16, 64>(W, src, so + pos, so + min(wend + 32, Ls), lane_id());
+    wave_sync();
+    const WalkRes r =
+        walk_range(reinterpret_cast<const uint32_t*>(W), wabs, so, Ls, pos, wend, meta, start, out + n);
+    wave_sync();  // W is restaged next
+    n += r.n;
+    pos = r.x;
+    if (r.st != kStLand) return {pos, n, r.st};
+  }
+  return {pos, n, kStLand};
+}
+
+// Walks chunk k of a long stream from pos (a wave-collective call).
+__device__ __forceinline__ WalkRes walk_chunk(uint4* W, const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls,
+                                              uint64_t k, uint64_t pos, uint32_t* __restrict__ meta,
+                                              uint64_t* __restrict__ start, uint64_t out) {
+  return walk_span(W, src, so, Ls, pos, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, out);
+}
+
+__device__ __forceinline__ uint64_t stream_len(const uint64_t* stream_off, uint64_t s, uint64_t* so) {
+  const uint64_t a = stream_off[s], b = stream_off[s + 1];
+  *so = a;
+  return b > a ? b - a : 0;
+}
+
+// The stream owning chunk c: the largest s < ns with chunk_base[s] <= c.
+__device__ uint64_t owner(const uint64_t* chunk_base, uint64_t ns, uint64_t c) {
+  uint64_t lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (chunk_base[mid] <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A stream is walked chunk-parallel when it is long and its chunks fit the
+// record scratch (max_chunks); every kernel applies this same test.
+__device__ __forceinline__ bool is_long(uint64_t Ls, const uint64_t* chunk_base, uint64_t s, uint64_t max_chunks) {
+  return Ls >= kLong && chunk_base[s + 1] <= max_chunks;
+}
+
+__device__ __forceinline__ int32_t status_of(uint32_t st) {
+  return st >= kStStop ? (int32_t)(st & 0xFFu) : UTL_OK;
+}
+
+// Chunks per stream (0 for short streams), scanned in place by k_walk_scan.
+__global__ void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns, uint64_t* __restrict__ q) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  uint64_t so;
+  const uint64_t Ls = s < ns ? stream_len(stream_off, s, &so) : 0;
+  q[s] = Ls >= kLong ? (Ls + kChunk - 1) / kChunk : 0;
+}
+
+// Exclusive scan of q[0..n] in place (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
+  __shared__ uint64_t part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t a = min((uint64_t)threadIdx.x * per, n), b = min(a + per, n);
+  uint64_t sum = 0;
+  for (uint64_t i = a; i < b; ++i) sum += q[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const uint64_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[threadIdx.x] - sum;
+  for (uint64_t i = a; i < b; ++i) {
+    const uint64_t v = q[i];
+    q[i] = run;
+    run += v;
+  }
+}
+
+// Entry vote of chunk k >= 1 (one workgroup of 256 threads per chunk).
+__global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ src,
+                                                   const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                   const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                   WalkRec* __restrict__ recs) {
+  __shared__ uint4 W[kWarm16];
+  __shared__ uint16_t J[kWarm];
+  __shared__ uint32_t hist[kLand];
+  __shared__ uint32_t best[4];
+  const uint64_t total = min(chunk_base[ns], max_chunks);
+  const int tid = threadIdx.x;
+  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint64_t s = owner(chunk_base, ns, c);
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint64_t k = c - chunk_base[s];
+    if (k == 0) {
+      if (tid == 0) recs[c].e = 0;
+      continue;
+    }
+    const uint64_t cs = k * (uint64_t)kChunk;  // chunk start (stream-relative)
+    const uint64_t ws = cs - kWarm;            // warm-up start
+    const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
+    for (int i = tid; i < kLand; i += 256) hist[i] = 0;
+    __syncthreads();
+    const uint32_t* W32 = reinterpret_cast<const uint32_t*>(W);
+    // first hop of every warm-up position: another warm-up position, a
+    // crossing into the chunk (kWarm + offset), or dead
+    for (int p = tid; p < kWarm; p += 256) {
+      uint32_t h, d1, d2;
+      win96(W32, (int)(so + ws + p - wabs), h, d1, d2);
+      const int64_t left = (int64_t)(Ls - (ws + p));
+      uint32_t bl;
+      const int err = step_len(h, d1, d2, left, &bl);
+      uint16_t j = kDead;
+      if (err == UTL_OK && (int64_t)bl < left) {  // bl == left ends the stream: no entry in this chunk
+        const uint64_t q = (uint64_t)p + bl;
+        if (q < (uint64_t)kWarm) j = (uint16_t)q;
+        else if (q - kWarm < (uint64_t)kLand) j = (uint16_t)q;
+      }
+      J[p] = j;
+    }
+    __syncthreads();
+    // pointer jumping: each hop doubles; (index, value) updates are single
+    // 16-bit words, so a stale read is still a position on the same path
+    constexpr int kPer = kWarm / 256;
+    for (;;) {
+      uint32_t v[kPer], t[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) v[u] = J[u * 256 + tid];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) t[u] = v[u] < (uint32_t)kWarm ? J[v[u]] : v[u];
+      int changed = 0;
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        if (t[u] != v[u]) {
+          J[u * 256 + tid] = (uint16_t)t[u];
+          changed = 1;
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+    for (int o = tid; o < kLand; o += 256) {
+      const uint16_t v = J[o];
+      if (v != kDead) atomicAdd(&hist[v - kWarm], 1u);
+    }
+    __syncthreads();
+    // most votes; ties to the smallest offset
+    uint32_t b = 0;
+    for (int i = tid; i < kLand; i += 256) {
+      const uint32_t key = hist[i] ? ((hist[i] << 16) | (uint32_t)(0xFFFF - i)) : 0u;
+      b = max(b, key);
+    }
+    b = wave_reduce_max(b);
+    if (lane_id() == 0) best[tid >> 6] = b;
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t m = max(max(best[0], best[1]), max(best[2], best[3]));
+      recs[c].e = m ? cs + (uint64_t)(0xFFFF - (m & 0xFFFF)) : kNone;
+    }
+    __syncthreads();
+  }
+}
+
+// Walk of every chunk from its voted entry (one wavefront per chunk).
+__global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
+                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                    WalkRec* __restrict__ recs) {
+  __shared__ uint4 W[kWin16];
+  const uint64_t total = min(chunk_base[ns], max_chunks);
+  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint64_t s = owner(chunk_base, ns, c);
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint64_t k = c - chunk_base[s];
+    const uint64_t e = recs[c].e;
+    WalkRes r{kNone, 0u, kStLand};
+    if (e != kNone) r = walk_chunk(W, src, so, Ls, k, e, nullptr, nullptr, 0);
+    if (lane_id() == 0) {
+      recs[c].x = r.x;
+      recs[c].n = r.n;
+      recs[c].st = r.st;
+    }
+  }
+}
+
+// Verification and block numbering of every long stream (one workgroup of
+// 1024 threads per stream, 1024 chunks per round). term[s] is the index of
+// the stream's last live chunk.
+constexpr int kVer = 1024;
+
+// Minimum and sum over the workgroup (16 wavefronts); red holds 32 words.
+__device__ __forceinline__ uint32_t wg_min(uint32_t x, uint32_t* red) {
+  x = wave_reduce_min(x);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t m = red[0];
+  for (int w = 1; w < kVer / 64; ++w) m = min(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+__global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict__ src,
+                                                      const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                      const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                      WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
+                                                      uint32_t* __restrict__ nblocks, int32_t* __restrict__ status) {
+  __shared__ uint4 W[kWin16];
+  __shared__ uint64_t xs[kVer];
+  __shared__ uint32_t sts[kVer];
+  __shared__ uint32_t red[32];
+  __shared__ uint64_t fix_x;
+  __shared__ uint32_t fix_n, fix_st;
+  const int tid = threadIdx.x, wid = tid >> 6;
+  for (uint64_t s = blockIdx.x; s < ns; s += gridDim.x) {
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint64_t c0 = chunk_base[s];
+    const uint64_t nch = chunk_base[s + 1] - c0;
+    uint64_t E = 0;  // true entry of chunk k
+    uint64_t k = 0;
+    uint32_t total = 0, last = (uint32_t)(nch - 1), st = kStStop | UTL_ERR_INVALID_BUFFER;
+    while (k < nch) {
+      const uint64_t j = k + (uint64_t)tid;
+      const bool in = j < nch;
+      WalkRec r{kNone, kNone, 0u, kStLand, 0u, 0u};
+      if (in) r = recs[c0 + j];
+      xs[tid] = r.x;
+      sts[tid] = r.st;
+      __syncthreads();
+      const uint64_t prev = tid == 0 ? E : xs[tid - 1];
+      const uint32_t fb = wg_min(in && r.e != prev ? (uint32_t)tid : (uint32_t)kVer, red);
+      const uint32_t ft = wg_min(in && r.st != kStLand ? (uint32_t)tid : (uint32_t)kVer, red);
+      const uint32_t ngood = fb <= ft ? fb : ft + 1;  // chunks k .. k + ngood - 1 are verified
+      const uint32_t cnt = (uint32_t)tid < ngood && in ? r.n : 0u;
+      const uint32_t incl = wave_incl_scan(cnt);
+      if (lane_id() == 63) red[wid] = incl;
+      __syncthreads();
+      uint32_t before = 0, sum = 0;
+      for (int w = 0; w < kVer / 64; ++w) {
+        before += w < wid ? red[w] : 0u;
+        sum += red[w];
+      }
+      if ((uint32_t)tid < ngood && in) recs[c0 + j].base = total + before + incl - cnt;
+      total += sum;
+      if (ft < (uint32_t)kVer && ft < fb) {  // verified up to a stop or the stream end
+        last = (uint32_t)(k + ft);
+        st = sts[ft];
+        __syncthreads();
+        break;
+      }
+      if (fb == (uint32_t)kVer) {  // all verified, none final
+        const bool more = nch - k > (uint64_t)kVer;
+        E = xs[kVer - 1];
+        k += kVer;
+        __syncthreads();
+        if (!more) break;  // cannot happen: a stream's last chunk always ends the walk
+        continue;
+      }
+      // chunk k + fb has a wrong entry: walk it again from the true one
+      E = fb == 0 ? E : xs[fb - 1];
+      k += fb;
+      if (wid == 0) {
+        const WalkRes w = walk_chunk(W, src, so, Ls, k, E, nullptr, nullptr, 0);
+        if (lane_id() == 0) {
+          WalkRec f{E, w.x, w.n, w.st, total, 0u};
+          recs[c0 + k] = f;
+          fix_x = w.x;
+          fix_n = w.n;
+          fix_st = w.st;
+        }
+      }
+      __syncthreads();
+      total += fix_n;
+      const uint32_t fst = fix_st;
+      E = fix_x;
+      __syncthreads();
+      if (fst != kStLand) {
+        last = (uint32_t)k;
+        st = fst;
+        break;
+      }
+      ++k;
+    }
+    if (tid == 0) {
+      term[s] = last;
+      nblocks[s] = total;
+      status[s] = status_of(st);
+    }
+  }
+}
+
+// Meta words and block starts of every live chunk (one wavefront per chunk).
+__global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ src,
+                                                  const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                  const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                  const WalkRec* __restrict__ recs, const uint32_t* __restrict__ term,
+                                                  const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
+                                                  uint64_t* __restrict__ start) {
+  __shared__ uint4 W[kWin16];
+  const uint64_t total = min(chunk_base[ns], max_chunks);
+  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+    const uint64_t s = owner(chunk_base, ns, c);
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint64_t k = c - chunk_base[s];
+    if (k > term[s]) continue;
+    const WalkRec r = recs[c];
+    if (r.n == 0) continue;
+    walk_chunk(W, src, so, Ls, k, r.e, meta, start, blk_off[s] + r.base);
+  }
+}
+
+// Serial walk of the short streams (one wavefront per stream).
+__global__ __launch_bounds__(64) void k_walk_serial(const uint8_t* __restrict__ src,
+                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                    const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
+                                                    uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
+                                                    int32_t* __restrict__ status) {
+  __shared__ uint4 W[kWin16];
+  for (uint64_t s = blockIdx.x; s < ns; s += gridDim.x) {
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (chunk_base && is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const WalkRes r = walk_span(W, src, so, Ls, 0, Ls, meta, start, meta ? blk_off[s] : 0);
+    if (lane_id() == 0) {
+      nblocks[s] = r.n;
+      status[s] = status_of(r.st);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace utl
+
+using namespace utl;
+
+size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
+  const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
+  return ((nstreams + 1) * 8 + 255) / 256 * 256 + max_chunks * sizeof(WalkRec) + nstreams * 4 + 256;
+}
+
+hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
+                                 uint64_t total_bytes, const uint64_t* blk_off, uint32_t* meta, uint64_t* start,
+                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream) {
+  if (nstreams == 0) return hipSuccess;
+  const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
+  uint8_t* t = static_cast<uint8_t*>(tmp);
+  uint64_t* chunk_base = reinterpret_cast<uint64_t*>(t);
+  t += ((nstreams + 1) * 8 + 255) / 256 * 256;
+  WalkRec* recs = reinterpret_cast<WalkRec*>(t);
+  t += max_chunks * sizeof(WalkRec);
+  uint32_t* term = reinterpret_cast<uint32_t*>(t);
+  const bool chunked = total_bytes >= kLong;
+  const unsigned sgrid = (unsigned)min(nstreams, (uint64_t)16384);
+  if (chunked) {
+    k_walk_plan<<<dim3((unsigned)((nstreams + 1 + 255) / 256)), dim3(256), 0, stream>>>(stream_off, nstreams,
+                                                                                         chunk_base);
+    k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(chunk_base, nstreams + 1);
+    const unsigned cgrid = (unsigned)min(max_chunks, (uint64_t)16384);
+    k_walk_vote<<<dim3(min(cgrid, 4096u)), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base,
+                                                                    max_chunks, recs);
+    k_walk_chunks<<<dim3(cgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs);
+    k_walk_verify<<<dim3(min(sgrid, 4096u)), dim3(kVer), 0, stream>>>(src, stream_off, nstreams, chunk_base,
+                                                                       max_chunks, recs, term, nblocks, status);
+    if (meta)
+      k_walk_emit<<<dim3(cgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs,
+                                                         term, blk_off, meta, start);
+  }
+  k_walk_serial<<<dim3(sgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
+                                                       max_chunks, blk_off, meta, start, nblocks, status);
+  return hipGetLastError();
+}

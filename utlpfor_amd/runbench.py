@@ -79,6 +79,8 @@ def walk_delta(ctx: Context, n: int = 64 << 20, progress=None):
         say(f"walk: {cuts.size - 1} streams of up to {per} blocks")
         rec = {"workload": f"foreign-stream walk, delta postings {n} u32, streams of up to {per} blocks"}
         rec.update(walk(ctx, s, boff[cuts]))
+        if per == s.nblocks:
+            rec["stream_bytes"] = s.bytes_host()  # for a host-side baseline; not JSON
         yield rec
     s.free()
 
