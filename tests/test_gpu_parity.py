@@ -405,3 +405,66 @@ def test_batched_u64(gpu, orc):
             wv, we = orc.GetUint64(int(pos[i] - ooff[k]), blob)
             assert gsn[i] == we and (we or gv[i] == wv)
         s.free()
+
+
+def _u64_far_second(orc, rng, pad):
+    """A uint64 two-block block whose first sub-block's svbLen is inflated by
+    `pad` zero bytes, so its second sub-block starts `pad` bytes later (the
+    reference accepts the longer StreamVByte area: exceptions.go:32-88)."""
+    lo = rng.integers(0, 1 << 10, 128, dtype=np.uint64)
+    k = rng.choice(128, 12, replace=False)
+    lo[k] = rng.integers(1 << 24, 1 << 32, 12, dtype=np.uint64)
+    hi = rng.integers(0, 1 << 32, 128, dtype=np.uint64)
+    v = (hi << np.uint64(32)) | lo
+    b = bytearray(orc.PackUint64(0, v)[0])
+    h = int.from_bytes(b[:4], "little")
+    assert (h >> 19) & 1 and h >> 24, "needs a two-block block with exceptions in its first sub-block"
+    forb = (1 << ((h >> 15) & 3)) >> 1
+    b2len = int.from_bytes(b[6 + forb:8 + forb], "little")
+    c1 = len(b) - b2len
+    out = b[:c1] + bytes(pad) + b[c1:]
+    out[4:6] = (int.from_bytes(b[4:6], "little") + pad).to_bytes(2, "little")
+    return v, bytes(out)
+
+
+def test_u64_second_subblock_outside_staged_extent(gpu, orc):
+    """Second sub-blocks that start past the staged extent of their block
+    are restaged one at a time; several in one wavefront, between ordinary
+    blocks, plus ones whose second sub-block then fails."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    codec, ctx = gpu
+    rng = np.random.default_rng(31)
+    blobs = []
+    for i in range(24):
+        if i % 3 == 2:
+            blobs.append(orc.PackUint64(0, rng.integers(0, 1 << 64, 128, dtype=np.uint64))[0])
+            continue
+        _, b = _u64_far_second(orc, rng, int(rng.integers(1400, 3000)))
+        if i % 5 == 4:  # corrupt the second sub-block's count (count mismatch / errors)
+            bb = bytearray(b)
+            hs = int.from_bytes(bb[:4], "little")
+            forb = (1 << ((hs >> 15) & 3)) >> 1
+            b2 = int.from_bytes(bb[6 + forb:8 + forb], "little")
+            bb[len(bb) - b2] = 77
+            b = bytes(bb)
+        blobs.append(b)
+    for b in blobs[:6]:
+        check_unpack(codec, ctx, orc, b, True)
+    data = np.frombuffer(b"".join(blobs), np.uint8)
+    lens = np.array([len(b) for b in blobs], np.uint64)
+    boff = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    ooff = (np.arange(len(blobs) + 1) * 128).astype(np.uint64)
+    s = batch.from_host_blocks(ctx, data, lens.astype(np.uint32), np.full(len(blobs), 128, np.uint32), is64=True)
+    cons = DeviceArray(len(blobs), np.int32, ctx)
+    out, st = batch.decode(ctx, s, consumed=cons)
+    ctx.sync()
+    wout, wst = orc.unpack_blocks(data, boff, ooff, 128 * len(blobs), True)
+    gst = st.to_numpy()
+    assert np.array_equal(gst, np.where((wst & 0xFF) == 99, 1, wst)), (gst, wst)
+    assert np.any(gst == 0) and np.any(gst != 0)
+    gout, gcons = out.to_numpy(), cons.to_numpy()
+    for k in np.nonzero(wst == 0)[0]:
+        assert np.array_equal(gout[128 * k:128 * k + 128], wout[128 * k:128 * k + 128]), k
+        assert gcons[k] == lens[k], k
+    s.free()

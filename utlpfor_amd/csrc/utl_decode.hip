@@ -31,7 +31,11 @@ constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 // Bytes of a sub-block the decoder can read: header, svbLen, FOR64 base,
 // block2Len (16) + payload (512) + bitmap (16) + control (32) + data (512).
 constexpr int kSubExtent = 1088;
-constexpr int kArena = 2560;        // staged bytes per wavefront (>= 15 + 2176 + 4)
+// Staged bytes per wavefront: u32 blocks use 2560 (>= 15 + 2176 + 4); u64
+// blocks 4608, which holds four two-block blocks of bw 32 per round and
+// costs no workgroup per CU (that kernel is held to 4 by its registers).
+constexpr int kArena32 = 2560;
+constexpr int kArena64 = 4608;
 constexpr int kCtrlMax = kWaveBlocks * 32;  // StreamVByte control bytes of a wave's blocks
 
 // Exceptions of the wave's blocks are numbered flat: block b's control bytes
@@ -40,14 +44,31 @@ constexpr int kCtrlMax = kWaveBlocks * 32;  // StreamVByte control bytes of a wa
 // aux. A block in position-list mode later reuses aux[128*b ..) to scatter.
 constexpr int kAux = kWaveBlocks * kBlock + 4;  // flat exception slots (+ room for reads past the last)
 
-struct alignas(16) WaveLds {
-  uint32_t arena[kArena / 4];
+// A tab entry holds a data offset into the arena (13 bits: every arena is
+// below 8 KiB), the block's bit width at bit 13 and the control byte at 24.
+constexpr uint32_t kTabOff = 0x1FFF;
+constexpr int kTabBwShift = 13;
+
+template <int ARENA>
+struct alignas(16) WaveLdsT {
+  static_assert(ARENA + 64 <= (int)kTabOff + 1, "tab offsets cover the arena");
+  static constexpr int kArenaBytes = ARENA;
+  uint32_t arena[ARENA / 4];
   uint32_t aux[kAux];       // exception high bits by flat slot
-  uint32_t tab[kCtrlMax];   // per control byte: data offset | bw << 12 | control byte << 24
+  uint32_t tab[kCtrlMax];   // per control byte: data offset | bw << 13 | control byte << 24
 };
-// 4 x 7696 B per workgroup: 5 workgroups per CU (one more byte of LDS per
-// wave costs a workgroup per CU).
-static_assert(sizeof(WaveLds) == 7696, "LDS per wavefront");
+// u32: 4 x 7696 B per workgroup, 5 workgroups per CU (one more byte of LDS
+// per wave costs a workgroup per CU). u64: 4 x 9744 B, 4 workgroups per CU.
+static_assert(sizeof(WaveLdsT<kArena32>) == 7696, "LDS per wavefront");
+static_assert(4 * 4 * sizeof(WaveLdsT<kArena64>) + 8 <= 160 * 1024, "four u64 workgroups per CU");
+template <bool U64>
+struct WaveLdsSel {
+  using type = WaveLdsT<kArena32>;
+};
+template <>
+struct WaveLdsSel<true> {
+  using type = WaveLdsT<kArena64>;
+};
 
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -228,12 +249,12 @@ __device__ __forceinline__ uint32_t svb_value(const uint32_t* a, uint32_t e, int
                                               uint32_t m_hi) {
   const int pre = slot + __popc(e & m_lo) + 2 * __popc(e & m_hi);
   const uint32_t code = (e >> (24 + 2 * slot)) & 3;
-  const int off = (int)(e & 0xFFF) + pre;
+  const int off = (int)(e & kTabOff) + pre;
   const int q = off >> 2;
   uint32_t v = __builtin_amdgcn_alignbyte(a[q + 1], a[q], off);  // uses off & 3
   const uint32_t drop = (code ^ 3) << 3;                          // bytes past the value's length
   v = (v << drop) >> drop;
-  return (uint32_t)((uint64_t)v << ((e >> 12) & 63));
+  return (uint32_t)((uint64_t)v << ((e >> kTabBwShift) & 63));
 }
 
 // Decodes one staged sub-block per 8-lane group whose first byte is at arena
@@ -241,8 +262,8 @@ __device__ __forceinline__ uint32_t svb_value(const uint32_t* a, uint32_t e, int
 // part in the wave-wide steps. blk is the group's index in the wave, j the
 // thread's index in the group. Returns the status word; count and consumed
 // are zero unless the status is UTL_OK. Every lane of the wave must call it.
-template <bool ForU64>
-__device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int at, int64_t len, int j,
+template <bool ForU64, class WL>
+__device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, int64_t len, int j,
                                               uint32_t* v0, uint32_t* v1, SubResult& res) {
   const uint32_t* a = W->arena;
   SubPlan p{};  // fields the prelude does not reach stay 0
@@ -283,7 +304,7 @@ __device__ __forceinline__ int32_t decode_sub(WaveLds* W, bool act, int blk, int
   if (has_svb) consumed = svb_start + p.svb_len - at;
   // Data offset of each owned control byte, with its bw and the byte.
   const uint32_t off = (uint32_t)(svb_start + ctrl + (int)incl - part);
-  const uint32_t bwf = (uint32_t)p.h.bw << 12;
+  const uint32_t bwf = (uint32_t)p.h.bw << kTabBwShift;
   uint4 te;
   te.x = (off + ((pre >> 0) & 0xFF)) | bwf | ((cw & 0xFFu) << 24);
   te.y = (off + min(4, nv) + ((pre >> 8) & 0xFF)) | bwf | (((cw >> 8) & 0xFFu) << 24);
@@ -485,38 +506,30 @@ __device__ __forceinline__ int64_t clamp_extent(int64_t len) {
 // staged range costs one round trip to HBM.
 // The chunks of one staged range, loaded into registers (scalar members: a
 // uint4 array here is placed in scratch).
+template <int N>
 struct Stage {
-  uint4 c0, c1, c2;
+  uint4 c[N];
   int nchunks;
 };
 
-__device__ __forceinline__ Stage stage_issue(const uint8_t* __restrict__ g, int64_t n) {
-  static_assert((kArena / 16 + 63) / 64 == 3, "three chunks per thread");
+template <class WL>
+__device__ __forceinline__ void stage_range(WL* W, const uint8_t* __restrict__ g, int64_t n) {
+  constexpr int N = (WL::kArenaBytes / 16 + 63) / 64;  // chunks per thread
   const int s = (int)(reinterpret_cast<uintptr_t>(g) & 15);
   const uint4* base = reinterpret_cast<const uint4*>(g - s);
-  Stage st;
+  Stage<N> st;
   st.nchunks = n > 0 ? (int)((s + n + 15) >> 4) : 0;
   const int l = lane_id();
-  st.c0 = make_uint4(0, 0, 0, 0);
-  st.c1 = st.c0;
-  st.c2 = st.c0;
-  if (l < st.nchunks) st.c0 = base[l];
-  if (l + 64 < st.nchunks) st.c1 = base[l + 64];
-  if (l + 128 < st.nchunks) st.c2 = base[l + 128];
-  return st;
-}
-
-__device__ __forceinline__ void stage_commit(WaveLds* W, const Stage& st) {
-  const int l = lane_id();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    st.c[k] = make_uint4(0, 0, 0, 0);
+    if (l + 64 * k < st.nchunks) st.c[k] = base[l + 64 * k];
+  }
   uint4* ar = reinterpret_cast<uint4*>(W->arena);
-  if (l < st.nchunks) ar[l] = st.c0;
-  if (l + 64 < st.nchunks) ar[l + 64] = st.c1;
-  if (l + 128 < st.nchunks) ar[l + 128] = st.c2;
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (l + 64 * k < st.nchunks) ar[l + 64 * k] = st.c[k];
   wave_fence();
-}
-
-__device__ __forceinline__ void stage_range(WaveLds* W, const uint8_t* __restrict__ g, int64_t n) {
-  stage_commit(W, stage_issue(g, n));
 }
 
 __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t out0, int count, int j,
@@ -557,84 +570,25 @@ __device__ __forceinline__ void store_u64(uint64_t* __restrict__ out, uint64_t o
   }
 }
 
-// State of one uint64 two-block block between its sub-block decodes.
-struct U64Pending {
-  int c1, b2len;  // block1Consumed and block2Len
-  bool pending;   // sub-block 2 lies outside the staged extent
-};
-
-// uint64.go:191-266 unpackUint64Block up to the second sub-block of a
-// two-block block, which is decoded here too when it is staged. Returns the
-// status. Every lane of the wave must call it (act selects the groups that
-// decode).
-__device__ __forceinline__ int32_t decode_u64_block(WaveLds* W, bool act, int blk, int at, int64_t len,
-                                                    int64_t avail, int j, uint32_t* l0, uint32_t* l1, uint32_t* h0,
-                                                    uint32_t* h1, uint64_t& base, bool& two, int& count,
-                                                    int& consumed, U64Pending& pend) {
-  two = false;
+// uint64.go:191-224 unpackUint64Block up to its first sub-block: length and
+// integer type, and the FOR64 base of a single block. Returns the status;
+// go is set when the first sub-block is to be decoded.
+__device__ __forceinline__ int32_t u64_prelude(const uint32_t* a, bool act, int at, int64_t len, Hdr& h,
+                                               uint64_t& base, bool& go) {
+  h = decode_header(0);
   base = 0;
-  count = 0;
-  consumed = 0;
-  pend.pending = false;
-  int32_t st = UTL_OK;
-  bool go1 = act;
-  Hdr h = decode_header(0);
-  if (go1) {
-    if (len < kHdr) {
-      st = UTL_ERR_INVALID_BUFFER;
-    } else {
-      h = decode_header(lds_u32(W->arena, at));
-      if (h.int_type != 3) {
-        st = UTL_ERR_UNSUPPORTED_TYPE;
-      } else if (!h.combine && is_for64_single(h.int_type, h.for_width, h.combine)) {
-        const int bo = kHdr + (h.exc > 0 ? kSvbLen : 0);
-        if (len < bo + kFor64)
-          st = UTL_ERR_INVALID_BUFFER;
-        else
-          base = (uint64_t)lds_u32(W->arena, at + bo) | ((uint64_t)lds_u32(W->arena, at + bo + 4) << 32);
-      }
-    }
-    go1 = st == UTL_OK;
+  go = false;
+  if (!act) return UTL_OK;
+  if (len < kHdr) return UTL_ERR_INVALID_BUFFER;
+  h = decode_header(lds_u32(a, at));
+  if (h.int_type != 3) return UTL_ERR_UNSUPPORTED_TYPE;
+  if (!h.combine && is_for64_single(h.int_type, h.for_width, h.combine)) {
+    const int bo = kHdr + (h.exc > 0 ? kSvbLen : 0);
+    if (len < bo + kFor64) return UTL_ERR_INVALID_BUFFER;
+    base = (uint64_t)lds_u32(a, at + bo) | ((uint64_t)lds_u32(a, at + bo + 4) << 32);
   }
-  SubResult r1;
-  const int32_t s1 = decode_sub<true>(W, go1, blk, at, len, j, l0, l1, r1);
-  bool go2 = false;
-  int b2len = 0;
-  if (go1) {
-    st = s1;
-    if (st == UTL_OK && !h.combine) {
-      count = r1.count;
-      consumed = r1.consumed;
-    } else if (st == UTL_OK) {
-      const bool he = h.exc > 0;
-      const int forb = for_base_bytes(h.for_width);
-      if (len < payload_offset(forb, he, true)) {
-        st = UTL_ERR_INVALID_BUFFER;
-      } else {
-        b2len = (int)lds_bytes(W->arena, at + kHdr + (he ? kSvbLen : 0) + forb, 2);
-        if ((int64_t)r1.consumed + b2len > len) {
-          st = UTL_ERR_INVALID_BUFFER;
-        } else {
-          two = true;
-          count = r1.count;
-          pend.c1 = r1.consumed;
-          pend.b2len = b2len;
-          if ((int64_t)r1.consumed + clamp_extent<false>(b2len) > avail)
-            pend.pending = true;
-          else
-            go2 = true;
-        }
-      }
-    }
-  }
-  SubResult r2;
-  const int32_t s2 = decode_sub<true>(W, go2, blk, at + (go2 ? r1.consumed : 0), b2len, j, h0, h1, r2);
-  if (go2) {
-    st = s2;
-    if (st == UTL_OK && r2.count != r1.count) st = UTL_ERR_INVALID_BUFFER;
-    if (st == UTL_OK) consumed = r1.consumed + b2len;
-  }
-  return st;
+  go = true;
+  return UTL_OK;
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
@@ -647,7 +601,8 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
 // stages as many consecutive blocks as fit the arena, decodes them, and
 // repeats until every block is done.
 template <bool U64>
-__device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
+__device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, const uint8_t* __restrict__ src,
+                                             const uint64_t* __restrict__ boff,
                              const uint64_t* __restrict__ ooff, uint64_t b0, int nb, void* __restrict__ out,
                              int32_t* __restrict__ status, int32_t* __restrict__ consumed_out) {
   const int blk = lane_id() >> 3, j = lane_id() & 7;
@@ -664,6 +619,7 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
   const uint64_t o_first = readlane64(my_o0, 0);
   const uint64_t o_lastm1 = readlane64(my_o0, last_lane);
   const int64_t ext_all = clamp_extent<U64>((int64_t)(readlane64(my_o1, last_lane) - o_lastm1));
+  constexpr int kArena = WaveLdsSel<U64>::type::kArenaBytes;
   const bool one_round = mono && (int64_t)(o_lastm1 - o_first) + ext_all + 15 + 4 <= kArena;
   int first = 0;
 #pragma unroll 1
@@ -714,34 +670,77 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
       }
       wave_fence();
     } else {
+      // uint64.go:191-266. One decode_sub instance serves every pass: pass 0
+      // decodes first sub-blocks, pass 1 the staged second sub-blocks of
+      // two-block blocks, and each further pass one second sub-block that
+      // lies outside the staged extent (corrupt blocks only), restaged.
+      // decode_sub writes h only on the lanes that decode in a pass, so h
+      // carries each block's second sub-block across the later passes.
       uint32_t l0[kRows], l1[kRows], h0[kRows], h1[kRows];
       uint64_t base64 = 0;
-      bool two = false;
-      int count = 0;
-      U64Pending pend;
-      st = decode_u64_block(W, active, bb, active ? at : 0, len, clamp_extent<true>(len), jj, l0, l1, h0, h1,
-                            base64, two, count, consumed, pend);
-      wave_fence();
-      // Corrupt blocks only: sub-block 2 outside the staged extent gets a
-      // staged round of its own, one block at a time.
-      uint64_t pmask = __ballot(active && st == UTL_OK && pend.pending && j == 0);
-      while (pmask) {
-        const int k = __ffsll((long long)pmask) - 1;  // lane of the block's thread 0
-        pmask &= pmask - 1;
-        const uint64_t kb = b0 + (k >> 3);
-        const int c1 = __shfl(pend.c1, k, 64), b2len = __shfl(pend.b2len, k, 64);
-        const uint8_t* g = src + boff[kb] + c1;
-        stage_range(W, g, clamp_extent<false>(b2len));
-        const bool me = bb == (k >> 3);
-        SubResult r2;
-        const int32_t s2 =
-            decode_sub<true>(W, me, bb, (int)(reinterpret_cast<uintptr_t>(g) & 15), b2len, jj, h0, h1, r2);
-        if (me) {
-          st = s2;
-          if (st == UTL_OK && r2.count != count) st = UTL_ERR_INVALID_BUFFER;
-          if (st == UTL_OK) consumed = c1 + b2len;
+      Hdr h;
+      bool go1;
+      st = u64_prelude(W->arena, active, active ? at : 0, len, h, base64, go1);
+      bool two = false, need2 = false, pending = false;
+      int count = 0, c1 = 0, b2len = 0;
+      {
+        SubResult r;
+        const int32_t s1 = decode_sub<true>(W, go1, bb, go1 ? at : 0, len, jj, l0, l1, r);
+        if (go1) {
+          st = s1;
+          if (st == UTL_OK && !h.combine) {
+            count = r.count;
+            consumed = r.consumed;
+          } else if (st == UTL_OK) {
+            const bool he = h.exc > 0;
+            const int forb = for_base_bytes(h.for_width);
+            if (len < payload_offset(forb, he, true)) {
+              st = UTL_ERR_INVALID_BUFFER;
+            } else {
+              b2len = (int)lds_bytes(W->arena, at + kHdr + (he ? kSvbLen : 0) + forb, 2);
+              if ((int64_t)r.consumed + b2len > len) {
+                st = UTL_ERR_INVALID_BUFFER;
+              } else {
+                two = true;
+                count = r.count;
+                c1 = r.consumed;
+                if ((int64_t)c1 + clamp_extent<false>(b2len) > clamp_extent<true>(len))
+                  pending = true;
+                else
+                  need2 = true;
+              }
+            }
+          }
         }
-        wave_fence();
+      }
+      if (__ballot(need2 || pending)) {
+        // pass 1: the staged second sub-blocks; then one pass per second
+        // sub-block outside the staged extent (corrupt blocks only), restaged
+        bool go_p = need2;
+        int at_p = need2 ? at + c1 : 0;
+        uint64_t pmask = __ballot(active && st == UTL_OK && pending && jj == 0);
+#pragma unroll 1
+        for (;;) {
+          int j2 = jj, b2 = bb;  // opaque per pass: keeps lane constants out of the loop
+          asm volatile("" : "+v"(j2), "+v"(b2));
+          SubResult r;
+          const int32_t s2 = decode_sub<true>(W, go_p, b2, go_p ? at_p : 0, b2len, j2, h0, h1, r);
+          wave_fence();
+          if (go_p) {
+            st = s2;
+            if (st == UTL_OK && r.count != count) st = UTL_ERR_INVALID_BUFFER;
+            if (st == UTL_OK) consumed = c1 + b2len;
+          }
+          if (pmask == 0) break;
+          const int k = __ffsll((long long)pmask) - 1;  // lane of the block's thread 0
+          pmask &= pmask - 1;
+          const uint64_t kb = b0 + (k >> 3);
+          const int ck = __shfl(c1, k, 64), bk = __shfl(b2len, k, 64);
+          const uint8_t* g = src + boff[kb] + ck;
+          stage_range(W, g, clamp_extent<false>(bk));
+          go_p = bb == (k >> 3);
+          at_p = (int)(reinterpret_cast<uintptr_t>(g) & 15);
+        }
       }
       if (active) {
         if (st == UTL_OK && (uint64_t)count > slot_n) st = UTL_ERR_ARGUMENT;
@@ -762,11 +761,11 @@ __device__ __forceinline__ void decode_group(WaveLds* W, const uint8_t* __restri
 __device__ __forceinline__ uint64_t xcd_tile(uint64_t g, uint64_t per_xcd) { return (g & 7) * per_xcd + (g >> 3); }
 
 template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
+__global__ __launch_bounds__(64 * kWgWaves, 4) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                                 const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                 void* __restrict__ out, int32_t* __restrict__ status,
                                                 int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
-  __shared__ WaveLds lds[kWgWaves];
+  __shared__ typename WaveLdsSel<U64>::type lds[kWgWaves];
   const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t tile = xcd_tile(blockIdx.x, per_xcd);
   const uint64_t b0 = tile * kTileBlocks + (uint64_t)wave * kWaveBlocks;
@@ -779,14 +778,14 @@ __global__ __launch_bounds__(64 * kWgWaves) void k_decode(const uint8_t* __restr
 // own work counter (counter[8x]) over XCD x's contiguous range of tiles, so
 // load balance stays inside the XCD. Every workgroup reaches the exit test.
 template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves) void k_decode_persistent(const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(64 * kWgWaves, 4) void k_decode_persistent(const uint8_t* __restrict__ src,
                                                            const uint64_t* __restrict__ boff,
                                                            const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                            void* __restrict__ out, int32_t* __restrict__ status,
                                                            int32_t* __restrict__ consumed_out,
                                                            unsigned long long* __restrict__ counter, uint64_t per_xcd,
                                                            uint32_t tiles_per_grab) {
-  __shared__ WaveLds lds[kWgWaves];
+  __shared__ typename WaveLdsSel<U64>::type lds[kWgWaves];
   __shared__ unsigned long long grab;
   const uint32_t xcd = blockIdx.x & 7;
   const uint32_t wave = uniform(threadIdx.x >> 6);
