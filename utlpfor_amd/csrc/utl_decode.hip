@@ -240,21 +240,15 @@ __device__ __forceinline__ void patch_pair(const uint32_t* x, uint32_t w, int b,
   vb |= ((hi & s0) | (lo & ~s0)) & s1;
 }
 
-// One StreamVByte value (svb.go:9-24, exceptions.go:32-88) shifted into the
-// high bits (exceptions.go:83-86, a shift by 32 gives 0): e is the table
-// entry of its control byte (data offset of the byte's first value, bw);
-// the codes of the slots before it (m_lo / m_hi select their low and high
-// bits in the entry) give the offset within.
-__device__ __forceinline__ uint32_t svb_value(const uint32_t* a, uint32_t e, int slot, uint32_t m_lo,
-                                              uint32_t m_hi) {
-  const int pre = slot + __popc(e & m_lo) + 2 * __popc(e & m_hi);
-  const uint32_t code = (e >> (24 + 2 * slot)) & 3;
-  const int off = (int)(e & kTabOff) + pre;
+// One StreamVByte value (svb.go:9-24, exceptions.go:32-88) of code `code`
+// (code + 1 bytes) at arena byte off, shifted into the high bits by bw
+// (exceptions.go:83-86; a shift by 32 gives 0).
+__device__ __forceinline__ uint32_t svb_at(const uint32_t* a, int off, uint32_t code, int bw) {
   const int q = off >> 2;
   uint32_t v = __builtin_amdgcn_alignbyte(a[q + 1], a[q], off);  // uses off & 3
   const uint32_t drop = (code ^ 3) << 3;                          // bytes past the value's length
   v = (v << drop) >> drop;
-  return (uint32_t)((uint64_t)v << ((e >> kTabBwShift) & 63));
+  return (uint32_t)((uint64_t)v << bw);
 }
 
 // Decodes one staged sub-block per 8-lane group whose first byte is at arena
@@ -325,22 +319,24 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
       if (c0 + 3 < ctrl_n) t[3] = te.w;
     }
     wave_fence();
-    // Slot f belongs to control byte f >> 2, code slot f & 3 (constant per
-    // lane since f advances by 64). Slots past a block's excCount, and the
-    // slots past the table of the last iteration, decode bytes nobody reads.
-    const int slot = lane_id() & 3;
-    const uint32_t pre_mask = ((1u << (2 * slot)) - 1u) << 24;  // codes of the slots before, in the entry
-    const uint32_t m_lo = pre_mask & 0x55000000u, m_hi = pre_mask & 0xAA000000u;
-    const int nslots = 4 * ctot;
+    // Lane c decodes the four values of control byte c (table entry c)
+    // into slots 4c .. 4c+3. Slots past a block's excCount decode bytes
+    // nobody reads.
 #pragma unroll 1
-    for (int f = lane_id(); f < nslots; f += 128) {
-      // slot f + 64 may lie past the table in the last iteration: its
-      // indexes are clamped and its value lands in a slot nobody reads
-      const int f1 = f + 64;
-      const uint32_t e0 = W->tab[f >> 2], e1 = W->tab[min(f1 >> 2, kCtrlMax - 1)];
-      const uint32_t x0 = svb_value(a, e0, slot, m_lo, m_hi), x1 = svb_value(a, e1, slot, m_lo, m_hi);
-      W->aux[f] = x0;
-      W->aux[min(f1, kAux - 1)] = x1;
+    for (int c = lane_id(); c < ctot; c += 64) {
+      const uint32_t e = W->tab[c];
+      const uint32_t codes = e >> 24;
+      const int bw = (int)((e >> kTabBwShift) & 63);
+      const int o0 = (int)(e & kTabOff);
+      const int o1 = o0 + (int)(codes & 3u) + 1;
+      const int o2 = o1 + (int)((codes >> 2) & 3u) + 1;
+      const int o3 = o2 + (int)((codes >> 4) & 3u) + 1;
+      uint4 x;
+      x.x = svb_at(a, o0, codes & 3u, bw);
+      x.y = svb_at(a, o1, (codes >> 2) & 3u, bw);
+      x.z = svb_at(a, o2, (codes >> 4) & 3u, bw);
+      x.w = svb_at(a, o3, codes >> 6, bw);
+      *reinterpret_cast<uint4*>(&W->aux[4 * c]) = x;
     }
     wave_fence();
   }
