@@ -253,6 +253,59 @@ int utl_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Plain device-to-device copy kernel (bandwidth calibration). */
 int utl_dev_copy(utl_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
+/* ---- stream container (no reference counterpart) ----
+ *
+ * The on-disk / wire form of a set of streams: the blocks exactly as
+ * PackUint32/PackUint64(flag|Append, ...) emit them, per 128-value chunk
+ * (pack.go:53-76), plus what the index needs and a stream directory:
+ *
+ *   [utl_container_header, 80 B]
+ *   [nstreams x utl_stream_entry, 48 B each]
+ *   [nblocks x u32 meta word (byte length | value count << 24), padded to 8 B]
+ *   [nbytes of concatenated blocks]
+ *
+ * All integers are little-endian. Block offsets are not stored: the device
+ * scan (utl_index_build) rebuilds them from the meta words. Host-only
+ * functions: no device work. */
+#define UTL_CONTAINER_MAGIC "UTLPFOR1"
+#define UTL_CONTAINER_VERSION 1u
+
+typedef struct utl_container_header {
+  char magic[8];         /* UTL_CONTAINER_MAGIC */
+  uint32_t version;      /* UTL_CONTAINER_VERSION */
+  uint32_t header_bytes; /* 80 */
+  uint64_t nstreams, nblocks, nbytes, nvalues;
+  uint64_t dir_off, meta_off, data_off, total_bytes;
+} utl_container_header;
+
+typedef struct utl_stream_entry {
+  uint64_t first_block; /* block range [first_block, first_block + nblocks) */
+  uint64_t nblocks;
+  uint64_t first_value; /* value range in the decoded concatenation */
+  uint64_t nvalues;
+  uint64_t byte_off;    /* first byte of the stream in the data section */
+  uint32_t flag;        /* the Pack flags the stream was written with */
+  uint32_t is_u64;      /* 0: uint32 blocks, 1: uint64 blocks */
+} utl_stream_entry;
+
+/* Bytes of a container holding these counts, or 0 on overflow. */
+size_t utl_container_bytes(size_t nstreams, size_t nblocks, size_t nbytes);
+
+/* Writes a container into dst (dst_cap >= utl_container_bytes(...)). The
+ * directory must tile [0, nblocks) and [0, nbytes) in order, and the meta
+ * words of each stream must sum to its bytes and values; UTL_ERR_ARGUMENT
+ * otherwise. */
+int utl_container_write(uint8_t *dst, size_t dst_cap, const utl_stream_entry *dir, size_t nstreams,
+                        const uint32_t *meta, size_t nblocks, const uint8_t *data, size_t nbytes,
+                        size_t *out_len);
+
+/* Validates a container held in src[0:len] (header, offsets, directory
+ * tiling, meta words against the directory) and returns its header;
+ * *dir, *meta and *data point into src. UTL_ERR_INVALID_BUFFER on any
+ * inconsistency. The blocks themselves are checked by decoding them. */
+int utl_container_parse(const uint8_t *src, size_t len, utl_container_header *hdr, const utl_stream_entry **dir,
+                        const uint32_t **meta, const uint8_t **data);
+
 /* Message of the last UTL_ERR_HIP / UTL_ERR_NO_DEVICE on this thread. */
 const char *utl_last_error(void);
 

@@ -223,3 +223,86 @@ def get_tune(ctx: Context):
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     check(_lib.load().utl_get_tune(ctx.handle, C.byref(a), C.byref(b), C.byref(c)), "utl_get_tune")
     return a.value, b.value, c.value
+
+
+# ---- stream container (utl_container_* in the C-ABI) ----
+
+STREAM_ENTRY = np.dtype([("first_block", "<u8"), ("nblocks", "<u8"), ("first_value", "<u8"), ("nvalues", "<u8"),
+                         ("byte_off", "<u8"), ("flag", "<u4"), ("is_u64", "<u4")])
+CONTAINER_HEADER = np.dtype([("magic", "S8"), ("version", "<u4"), ("header_bytes", "<u4"), ("nstreams", "<u8"),
+                             ("nblocks", "<u8"), ("nbytes", "<u8"), ("nvalues", "<u8"), ("dir_off", "<u8"),
+                             ("meta_off", "<u8"), ("data_off", "<u8"), ("total_bytes", "<u8")])
+assert STREAM_ENTRY.itemsize == 48 and CONTAINER_HEADER.itemsize == 80
+
+
+def directory(s: DeviceStream, stream_lengths, stream_flags=None) -> np.ndarray:
+    """Stream directory of a stream set encoded by encode() with these
+    per-stream value counts (and flags)."""
+    lens = np.asarray(stream_lengths, dtype=np.uint64)
+    blk = plan_blocks(lens)
+    boff = s.block_off.to_numpy()
+    d = np.zeros(lens.size, STREAM_ENTRY)
+    d["first_block"] = blk[:-1]
+    d["nblocks"] = np.diff(blk)
+    d["first_value"] = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint64)
+    d["nvalues"] = lens
+    d["byte_off"] = boff[blk[:-1].astype(np.int64)]
+    d["flag"] = 0 if stream_flags is None else np.asarray(stream_flags, dtype=np.uint32)
+    d["is_u64"] = int(s.is64)
+    return d
+
+
+def to_container(s: DeviceStream, dir_: np.ndarray) -> np.ndarray:
+    """The container bytes (uint8 array) of a device stream set."""
+    lib = _lib.load()
+    dir_ = np.ascontiguousarray(dir_, dtype=STREAM_ENTRY)
+    meta = s.meta.to_numpy(s.nblocks) if s.nblocks else np.zeros(0, np.uint32)
+    data = s.bytes_host() if s.nbytes else np.zeros(0, np.uint8)
+    n = lib.utl_container_bytes(dir_.size, s.nblocks, s.nbytes)
+    if n == 0:
+        raise ValueError("container size overflows")
+    out = np.empty(n, np.uint8)
+    out_len = C.c_size_t()
+    check(lib.utl_container_write(out.ctypes.data, n, dir_.ctypes.data, dir_.size, meta.ctypes.data, s.nblocks,
+                                  data.ctypes.data, s.nbytes, C.byref(out_len)), "utl_container_write")
+    return out
+
+
+def parse_container(buf: np.ndarray):
+    """Validates container bytes; returns (header record, directory, meta
+    words, data bytes), the last three as views into buf."""
+    lib = _lib.load()
+    buf = np.asarray(buf, dtype=np.uint8)
+    if buf.ctypes.data % 8:
+        buf = buf.copy()  # sections are 8-byte aligned within the buffer
+    hdr = np.zeros(1, CONTAINER_HEADER)
+    pd, pm, px = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    check(lib.utl_container_parse(buf.ctypes.data, buf.size, hdr.ctypes.data, C.byref(pd), C.byref(pm),
+                                  C.byref(px)), "utl_container_parse")
+    h = hdr[0]
+    do, mo, xo = int(h["dir_off"]), int(h["meta_off"]), int(h["data_off"])
+    ns, nb, nx = int(h["nstreams"]), int(h["nblocks"]), int(h["nbytes"])
+    dir_ = buf[do:do + ns * STREAM_ENTRY.itemsize].view(STREAM_ENTRY)
+    meta = buf[mo:mo + 4 * nb].view(np.uint32)
+    return h, dir_, meta, buf[xo:xo + nx]
+
+
+def from_container(ctx: Context, buf: np.ndarray):
+    """Uploads a container whose streams are all uint32 or all uint64 and
+    builds its index on the device. Returns (DeviceStream, directory)."""
+    h, dir_, meta, data = parse_container(buf)
+    kinds = set(dir_["is_u64"].tolist())
+    if len(kinds) > 1:
+        raise ValueError("container mixes uint32 and uint64 streams; decode them by block range")
+    is64 = bool(kinds.pop()) if kinds else False
+    return from_host_blocks(ctx, data, meta & 0xFFFFFF, meta >> 24, is64), dir_.copy()
+
+
+def save_container(path: str, s: DeviceStream, dir_: np.ndarray) -> int:
+    buf = to_container(s, dir_)
+    buf.tofile(path)
+    return int(buf.size)
+
+
+def load_container(ctx: Context, path: str):
+    return from_container(ctx, np.fromfile(path, dtype=np.uint8))
