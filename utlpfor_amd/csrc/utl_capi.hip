@@ -49,8 +49,8 @@ int hip_fail(hipError_t e, const char* what) {
 struct utl_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint8_t* h_stage = nullptr;  // pinned
-  uint8_t* d_stage = nullptr;
+  uint8_t* h_stage = nullptr;  // pinned, host-mapped, fine-grained (coherent)
+  uint8_t* d_stage = nullptr;  // the device's address of h_stage
   void* d_tmp = nullptr;
   size_t d_tmp_bytes = 0;
   unsigned long long* d_counter = nullptr;
@@ -78,8 +78,12 @@ int ctx_init(utl_ctx* c, int device) {
   c->cu_count = prop.multiProcessorCount;
   snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_TRY(hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocDefault));
-  HIP_TRY(hipMalloc((void**)&c->d_stage, kStageBytes));
+  // Per-call staging: kernels read a call's inputs from and write its
+  // results to this host buffer directly (fine-grained, so no GPU cache
+  // holds a stale line between calls): one launch and one synchronization
+  // per call, no copies.
+  HIP_TRY(hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocMapped | hipHostMallocCoherent));
+  HIP_TRY(hipHostGetDevicePointer((void**)&c->d_stage, c->h_stage, 0));
   HIP_TRY(hipMalloc((void**)&c->d_counter, 4096));  // 8 per-XCD work counters, 64 B apart
   c->tune.grid_wgs = c->cu_count * 4;
   return UTL_OK;
@@ -91,7 +95,6 @@ void ctx_free(utl_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->d_tmp) hipFree(c->d_tmp);
   if (c->d_counter) hipFree(c->d_counter);
-  if (c->d_stage) hipFree(c->d_stage);
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -151,7 +154,6 @@ int stage_block(utl_ctx* c, const uint8_t* src, size_t len) {
   hdr[2] = 0;
   hdr[3] = 128;
   if (ncopy) memcpy(c->h_stage + 64, src, ncopy);
-  HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, 64 + ncopy, hipMemcpyHostToDevice, c->stream));
   return UTL_OK;
 }
 
@@ -249,9 +251,8 @@ static int unpack_call(utl_ctx* ctx, bool u64, const uint8_t* src, size_t len, v
   void* d_vals = d + kResultOff + 64;
   HIP_TRY(utl_launch_decode(u64, d + 64, (const uint64_t*)d, (const uint64_t*)(d + 16), 1, d_vals, d_status, d_cons,
                             c->stream));
-  HIP_TRY(hipMemcpyAsync(c->h_stage + kResultOff, d + kResultOff, 64 + 128 * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const int32_t st = *(int32_t*)(c->h_stage + kResultOff);
+  const int32_t st = *(volatile int32_t*)(c->h_stage + kResultOff);
   const int code = UTL_STATUS_CODE(st);
   if (code == UTL_OK) {
     // a decoded block holds its header count (0 for an empty block)
@@ -292,15 +293,13 @@ static int get_call(utl_ctx* ctx, bool u64, int64_t pos, const uint8_t* src, siz
   hdr[4] = 0;                // request block
   hdr[5] = (uint64_t)pos;    // request position
   if (ncopy) memcpy(c->h_stage + 64, src, ncopy);
-  HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, 64 + ncopy, hipMemcpyHostToDevice, c->stream));
   uint8_t* d = c->d_stage;
   int32_t* d_status = (int32_t*)(d + kResultOff);
   void* d_val = d + kResultOff + 64;
   HIP_TRY(utl_launch_get(u64, d + 64, (const uint64_t*)d, (const uint64_t*)(d + 16), 1, (const uint64_t*)(d + 32),
                          (const int64_t*)(d + 40), 1, d_val, d_status, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->h_stage + kResultOff, d + kResultOff, 72, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const int32_t st = *(int32_t*)(c->h_stage + kResultOff);
+  const int32_t st = *(volatile int32_t*)(c->h_stage + kResultOff);
   if (u64) *(uint64_t*)out = st == UTL_OK ? *(uint64_t*)(c->h_stage + kResultOff + 64) : 0;
   else *(uint32_t*)out = st == UTL_OK ? *(uint32_t*)(c->h_stage + kResultOff + 64) : 0;
   // An overflow inside Get's full-unpack fallback is still ErrOverflow.
@@ -333,7 +332,6 @@ static int pack_call(utl_ctx* ctx, bool u64, uint32_t flag, const void* values, 
   hdr[0] = 0;
   *(uint32_t*)(c->h_stage + 8) = (uint32_t)n | ((flag & 0xFF) << 8);
   memcpy(c->h_stage + 64, values, n * (u64 ? 8 : 4));
-  HIP_TRY(hipMemcpyAsync(c->d_stage, c->h_stage, 64 + n * (u64 ? 8 : 4), hipMemcpyHostToDevice, c->stream));
   uint8_t* d = c->d_stage;
   uint32_t* d_meta = (uint32_t*)(d + kResultOff);
   uint32_t* d_work = (uint32_t*)(d + kResultOff + 64);
@@ -344,9 +342,8 @@ static int pack_call(utl_ctx* ctx, bool u64, uint32_t flag, const void* values, 
   else
     HIP_TRY(utl_launch_encode_u32((const uint32_t*)(d + 64), (const uint64_t*)d, (const uint32_t*)(d + 8), 1, d_slot,
                                   1024, d_meta, d_work, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->h_stage + kResultOff, d + kResultOff, 1024 + 2048, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const uint32_t meta = *(uint32_t*)(c->h_stage + kResultOff);
+  const uint32_t meta = *(volatile uint32_t*)(c->h_stage + kResultOff);
   if (meta == 0xFFFFFFFFu) return UTL_ERR_INVALID_BUFFER;
   const size_t len = meta & 0xFFFFFF;
   if (out_len) *out_len = len;

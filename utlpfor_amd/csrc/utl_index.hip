@@ -4,8 +4,9 @@
 // BlockLength (block_length.go:8) of block k ends (example_test.go:171-187).
 // Here the per-block meta words (byte length | value count << 24) that the
 // encoders emit are turned into block byte offsets and output value offsets
-// by a three-pass device exclusive scan (tile reduce, tile-prefix scan, tile
-// apply), which removes the loop-carried dependency. A header-parse pass
+// by a device exclusive scan (tile reduce, tile-prefix scan, tile apply; the
+// middle pass is folded into the apply pass when there are few tiles),
+// which removes the loop-carried dependency. A header-parse pass
 // (k_block_lengths) recomputes BlockLength for every block whose start is
 // known, on the device.
 #include "utl_device.h"
@@ -117,6 +118,104 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
   }
 }
 
+// Coalesced form (16-byte aligned meta, block_off and out_off): tiles of
+// 1024 blocks, thread t owning blocks 4t .. 4t+3 of its tile, read with one
+// 16-byte load and written with 16-byte stores. With at most kInlineTiles
+// tiles the apply pass sums the earlier tiles' totals itself, which saves
+// the tile-prefix pass.
+constexpr int kTile4 = kScanThreads * 4;
+constexpr uint64_t kInlineTiles = 1024;
+
+__device__ __forceinline__ void load_meta4(const uint32_t* __restrict__ meta, uint64_t n, uint64_t i,
+                                           uint64_t* l, uint64_t* c) {
+  uint32_t m[4] = {0u, 0u, 0u, 0u};
+  if (i + 4 <= n) {
+    const uint4 q = *reinterpret_cast<const uint4*>(meta + i);
+    m[0] = q.x;
+    m[1] = q.y;
+    m[2] = q.z;
+    m[3] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + k < n) m[k] = meta[i + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) split_meta(m[k], l[k], c[k]);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce4(const uint32_t* __restrict__ meta, uint64_t n,
+                                                                uint64_t* __restrict__ tile_len,
+                                                                uint64_t* __restrict__ tile_cnt) {
+  __shared__ uint64_t lds[2][4];
+  uint64_t l[4], c[4];
+  load_meta4(meta, n, (uint64_t)blockIdx.x * kTile4 + 4 * threadIdx.x, l, c);
+  uint64_t tl, tc;
+  wg_excl_scan(l[0] + l[1] + l[2] + l[3], lds[0], tl);
+  wg_excl_scan(c[0] + c[1] + c[2] + c[3], lds[1], tc);
+  if (threadIdx.x == 0) {
+    tile_len[blockIdx.x] = tl;
+    tile_cnt[blockIdx.x] = tc;
+  }
+}
+
+// INLINE: tile_len/tile_cnt hold tile totals and each workgroup sums those
+// before it; otherwise they hold exclusive tile prefixes (k_scan_tiles).
+template <bool INLINE>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __restrict__ meta, uint64_t n,
+                                                               const uint64_t* __restrict__ tile_len,
+                                                               const uint64_t* __restrict__ tile_cnt,
+                                                               uint64_t* __restrict__ boff,
+                                                               uint64_t* __restrict__ ooff) {
+  __shared__ uint64_t lds[4][4];
+  const uint64_t i0 = (uint64_t)blockIdx.x * kTile4 + 4 * threadIdx.x;
+  uint64_t l[4], c[4];
+  load_meta4(meta, n, i0, l, c);
+  uint64_t tile_pl, tile_pc;
+  if (INLINE) {
+    uint64_t sl = 0, sc = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) {
+      sl += tile_len[t];
+      sc += tile_cnt[t];
+    }
+    wg_excl_scan(sl, lds[2], tile_pl);
+    wg_excl_scan(sc, lds[3], tile_pc);
+  } else {
+    tile_pl = tile_len[blockIdx.x];
+    tile_pc = tile_cnt[blockIdx.x];
+  }
+  uint64_t tl, tc;
+  uint64_t pl = wg_excl_scan(l[0] + l[1] + l[2] + l[3], lds[0], tl) + tile_pl;
+  uint64_t pc = wg_excl_scan(c[0] + c[1] + c[2] + c[3], lds[1], tc) + tile_pc;
+  uint64_t b[5], o[5];
+  b[0] = pl;
+  o[0] = pc;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    b[k + 1] = b[k] + l[k];
+    o[k + 1] = o[k] + c[k];
+  }
+  if (i0 + 4 <= n) {
+    *reinterpret_cast<ulonglong2*>(boff + i0) = make_ulonglong2(b[0], b[1]);
+    *reinterpret_cast<ulonglong2*>(boff + i0 + 2) = make_ulonglong2(b[2], b[3]);
+    *reinterpret_cast<ulonglong2*>(ooff + i0) = make_ulonglong2(o[0], o[1]);
+    *reinterpret_cast<ulonglong2*>(ooff + i0 + 2) = make_ulonglong2(o[2], o[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i0 + k < n) {
+        boff[i0 + k] = b[k];
+        ooff[i0 + k] = o[k];
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + k + 1 == n) {
+      boff[n] = b[k + 1];
+      ooff[n] = o[k + 1];
+    }
+}
+
 // BlockLength of every block with a known start (header-parse pass).
 __global__ void k_block_lengths(const uint8_t* __restrict__ src, const uint64_t* __restrict__ start,
                                 const uint64_t* __restrict__ limit, uint64_t n, uint32_t* __restrict__ meta,
@@ -153,7 +252,7 @@ __global__ void k_fill8(uint8_t* p, uint8_t v, uint64_t n) {
 using namespace utl;
 
 size_t utl_index_tmp_bytes(uint64_t nblocks) {
-  const uint64_t ntiles = (nblocks + kTile - 1) / kTile;
+  const uint64_t ntiles = (nblocks + kTile4 - 1) / kTile4;  // the smaller of the two tiles
   return (size_t)(2 * ntiles * sizeof(uint64_t) + 256);
 }
 
@@ -161,6 +260,19 @@ hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, ui
                             hipStream_t stream) {
   if (n == 0) {
     k_zero_first<<<1, 1, 0, stream>>>(boff, ooff);
+    return hipGetLastError();
+  }
+  if ((((uintptr_t)meta | (uintptr_t)boff | (uintptr_t)ooff) & 15) == 0) {
+    const uint64_t nt = (n + kTile4 - 1) / kTile4;
+    uint64_t* tl = (uint64_t*)tmp;
+    uint64_t* tc = tl + nt;
+    k_scan_reduce4<<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc);
+    if (nt <= kInlineTiles) {
+      k_scan_apply4<true><<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+    } else {
+      k_scan_tiles<<<1, kScanThreads, 0, stream>>>(tl, tc, nt);
+      k_scan_apply4<false><<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+    }
     return hipGetLastError();
   }
   const uint64_t ntiles = (n + kTile - 1) / kTile;
