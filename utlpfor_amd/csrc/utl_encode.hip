@@ -652,10 +652,13 @@ __global__ __launch_bounds__(64 * kEncWaves) void k_encode_u64(const uint64_t* _
 // Little-endian dword of slot bytes [i, i + 4) (slots are 4-byte aligned; the
 // second dword is read only when the bytes straddle it, so a read ending at
 // the slot's last byte stays inside the slot).
+// Little-endian dword at byte i of a slot (i <= slot bytes - 4). Both loads
+// are unconditional (an aligned i reads its own dword twice), so a caller's
+// loads can all be in flight together.
 __device__ __forceinline__ uint32_t slot_u32(const uint8_t* __restrict__ s, uint32_t i) {
   const uint32_t* w = reinterpret_cast<const uint32_t*>(s) + (i >> 2);
-  const uint32_t hi = (i & 3) ? w[1] : 0u;
-  return __builtin_amdgcn_alignbyte(hi, w[0], i & 3);
+  const uint32_t lo = w[0], hi = w[(i & 3) ? 1 : 0];
+  return __builtin_amdgcn_alignbyte(hi, lo, i & 3);
 }
 
 // Where a block starts in its slot: after the zero bytes the encoders put
@@ -669,32 +672,48 @@ __device__ __forceinline__ uint32_t slot_shift(const uint8_t* __restrict__ s) {
 // aligned dword stores: the output dword d is written by the block holding
 // byte 4d; blocks are at least 4 bytes, so its bytes past that block come
 // from the next block's slot. Eight lanes per block, eight blocks per
-// wavefront. dst must have room for the dword holding the last byte.
+// wavefront; each lane moves four dwords per round with all their slot
+// loads in flight before the first store. dst must have room for the dword
+// holding the last byte.
 __global__ __launch_bounds__(256) void k_compact(const uint8_t* __restrict__ slots, uint32_t slot_bytes,
                                                  const uint64_t* __restrict__ boff, uint64_t nblocks,
                                                  uint8_t* __restrict__ dst) {
   const uint64_t b = ((uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6)) * 8 + (lane_id() >> 3);
   if (b >= nblocks) return;
+  const int l8 = lane_id() & 7;
   const uint64_t o = boff[b], o1 = boff[b + 1];
   const uint8_t* s = slots + b * slot_bytes;
   const bool has_next = b + 1 < nblocks;
   const uint8_t* sn = slots + (has_next ? b + 1 : b) * slot_bytes;
-  const uint32_t sh = slot_shift(s), shn = slot_shift(sn);
+  const uint32_t sh = slot_shift(s);
+  // the first bytes of the next block, for the dword that straddles o1
+  const uint32_t next_w = has_next ? slot_u32(sn, slot_shift(sn)) : 0u;
   uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
   const uint64_t d0 = (o + 3) >> 2, d1 = (o1 + 3) >> 2;  // dwords whose first byte is in this block
-  for (uint64_t d = d0 + (lane_id() & 7); d < d1; d += 8) {
-    const uint64_t p = 4 * d;            // first byte of the dword, in [o, o1)
-    // sh + (p - o) <= 3 + MaxBlockLength - 1 for every block the encoders
-    // produce; the clamp keeps reads inside the slot for an index built over
-    // rejected blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
-    const uint32_t in = (uint32_t)min(p - o + sh, (uint64_t)(slot_bytes - 4));
-    uint32_t w = slot_u32(s, in);
-    const uint64_t mine = o1 - p;        // bytes of the dword in this block (>= 1)
-    if (mine < 4) {
-      const uint32_t keep = (1u << (8 * mine)) - 1u;
-      w = (w & keep) | (has_next ? (slot_u32(sn, shn) << (8 * mine)) : 0u);
+  const int ndw = (int)(d1 - d0);
+  // sh + (p - o) <= 3 + MaxBlockLength - 1 for every block the encoders
+  // produce; the clamp keeps reads inside the slot for an index built over
+  // rejected blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
+  const uint64_t in0 = 4 * d0 - o + sh;
+  for (int k0 = 0; k0 < ndw; k0 += 32) {
+    uint32_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 8 * u + l8;
+      const uint32_t in = (uint32_t)min(in0 + 4 * (uint64_t)k, (uint64_t)(slot_bytes - 4));
+      w[u] = slot_u32(s, in);  // past the block: a clamped in-slot read, not stored
     }
-    d32[d] = w;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 8 * u + l8;
+      if (k < ndw) {
+        const uint64_t p = 4 * (d0 + (uint64_t)k);  // first byte of the dword, in [o, o1)
+        const uint64_t mine = o1 - p;               // bytes of the dword in this block (>= 1)
+        uint32_t v = w[u];
+        if (mine < 4) v = (v & ((1u << (8 * mine)) - 1u)) | (next_w << (8 * mine));
+        d32[d0 + k] = v;
+      }
+    }
   }
 }
 
