@@ -36,30 +36,31 @@ constexpr int kSubExtent = 1088;
 // costs no workgroup per CU (that kernel is held to 4 by its registers).
 constexpr int kArena32 = 2560;
 constexpr int kArena64 = 4608;
-constexpr int kCtrlMax = kWaveBlocks * 32;  // StreamVByte control bytes of a wave's blocks
-
 // Exceptions of the wave's blocks are numbered flat: block b's control bytes
-// are entries [cbase_b, cbase_b + ctrl_b) of tab, with cbase_b the control
-// bytes of the blocks before it, and its exception i is slot 4*cbase_b + i of
-// aux. A block in position-list mode later reuses aux[128*b ..) to scatter.
+// are control bytes [cbase_b, cbase_b + ctrl_b) of the wave, with cbase_b the
+// control bytes of the blocks before it, and its exception i is slot
+// 4*cbase_b + i of aux. The table entry of control byte c waits in slot
+// 4c + 3 until lane c reads it and overwrites slots 4c .. 4c+3 with the four
+// values it decodes (no other lane touches them). A block in position-list
+// mode later reuses aux[128*b ..) to scatter.
 constexpr int kAux = kWaveBlocks * kBlock + 4;  // flat exception slots (+ room for reads past the last)
 
-// A tab entry holds a data offset into the arena (13 bits: every arena is
+// A table entry holds a data offset into the arena (13 bits: every arena is
 // below 8 KiB), the block's bit width at bit 13 and the control byte at 24.
 constexpr uint32_t kTabOff = 0x1FFF;
 constexpr int kTabBwShift = 13;
 
 template <int ARENA>
 struct alignas(16) WaveLdsT {
-  static_assert(ARENA + 64 <= (int)kTabOff + 1, "tab offsets cover the arena");
+  static_assert(ARENA + 64 <= (int)kTabOff + 1, "table offsets cover the arena");
   static constexpr int kArenaBytes = ARENA;
   uint32_t arena[ARENA / 4];
-  uint32_t aux[kAux];       // exception high bits by flat slot
-  uint32_t tab[kCtrlMax];   // per control byte: data offset | bw << 13 | control byte << 24
+  uint32_t aux[kAux];  // exception high bits by flat slot (and the table entries before that)
 };
-// u32: 4 x 7696 B per workgroup, 5 workgroups per CU (one more byte of LDS
-// per wave costs a workgroup per CU). u64: 4 x 9744 B, 4 workgroups per CU.
-static_assert(sizeof(WaveLdsT<kArena32>) == 7696, "LDS per wavefront");
+// u32: 4 x 6672 B per workgroup, 6 workgroups per CU; u64: 4 x 8720 B, 4
+// workgroups per CU (that kernel is held to 4 by its registers).
+static_assert(sizeof(WaveLdsT<kArena32>) == 6672, "LDS per wavefront");
+static_assert(6 * 4 * sizeof(WaveLdsT<kArena32>) + 8 <= 160 * 1024, "six u32 workgroups per CU");
 static_assert(4 * 4 * sizeof(WaveLdsT<kArena64>) + 8 <= 160 * 1024, "four u64 workgroups per CU");
 template <bool U64>
 struct WaveLdsSel {
@@ -312,19 +313,19 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
   if (ctot > 0) {
     if (ctrl_n > 0) {
       const int c0 = 4 * j;
-      uint32_t* t = &W->tab[cbase + c0];
+      uint32_t* t = &W->aux[4 * (cbase + c0) + 3];  // entry of control byte c in slot 4c + 3
       if (c0 < ctrl_n) t[0] = te.x;
-      if (c0 + 1 < ctrl_n) t[1] = te.y;
-      if (c0 + 2 < ctrl_n) t[2] = te.z;
-      if (c0 + 3 < ctrl_n) t[3] = te.w;
+      if (c0 + 1 < ctrl_n) t[4] = te.y;
+      if (c0 + 2 < ctrl_n) t[8] = te.z;
+      if (c0 + 3 < ctrl_n) t[12] = te.w;
     }
     wave_fence();
-    // Lane c decodes the four values of control byte c (table entry c)
-    // into slots 4c .. 4c+3. Slots past a block's excCount decode bytes
-    // nobody reads.
+    // Lane c decodes the four values of control byte c (its table entry in
+    // slot 4c + 3) into slots 4c .. 4c+3. Slots past a block's excCount
+    // decode bytes nobody reads.
 #pragma unroll 1
     for (int c = lane_id(); c < ctot; c += 64) {
-      const uint32_t e = W->tab[c];
+      const uint32_t e = W->aux[4 * c + 3];
       const uint32_t codes = e >> 24;
       const int bw = (int)((e >> kTabBwShift) & 63);
       const int o0 = (int)(e & kTabOff);
@@ -757,7 +758,7 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
 __device__ __forceinline__ uint64_t xcd_tile(uint64_t g, uint64_t per_xcd) { return (g & 7) * per_xcd + (g >> 3); }
 
 template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves, 4) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
+__global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 6) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                                 const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                 void* __restrict__ out, int32_t* __restrict__ status,
                                                 int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
