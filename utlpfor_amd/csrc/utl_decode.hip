@@ -40,10 +40,16 @@ constexpr int kArena64 = 4608;
 // are control bytes [cbase_b, cbase_b + ctrl_b) of the wave, with cbase_b the
 // control bytes of the blocks before it, and its exception i is slot
 // 4*cbase_b + i of aux. The table entry of control byte c waits in slot
-// 4c + 3 until lane c reads it and overwrites slots 4c .. 4c+3 with the four
-// values it decodes (no other lane touches them). A block in position-list
-// mode later reuses aux[128*b ..) to scatter.
+// kTabBase + c (consecutive entries: no LDS bank conflicts). Round k of the
+// decode loop reads entries [64k, 64k + 64) and then writes the values of
+// those control bytes to slots [256k, 256k + 256); those reach kTabBase + c
+// only for entries this round or an earlier one has already read, so the
+// table can share the slots. A block in position-list mode later reuses
+// aux[128*b ..) to scatter.
 constexpr int kAux = kWaveBlocks * kBlock + 4;  // flat exception slots (+ room for reads past the last)
+constexpr int kCtrlMax = kWaveBlocks * 32;       // control bytes of a wave's blocks
+constexpr int kTabBase = kWaveBlocks * kBlock - kCtrlMax;  // 768
+static_assert(4 * 64 * (kCtrlMax / 64 - 1) + 4 * 64 <= kTabBase + kCtrlMax, "table rounds");
 
 // A table entry holds a data offset into the arena (13 bits: every arena is
 // below 8 KiB), the block's bit width at bit 13 and the control byte at 24.
@@ -313,19 +319,19 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
   if (ctot > 0) {
     if (ctrl_n > 0) {
       const int c0 = 4 * j;
-      uint32_t* t = &W->aux[4 * (cbase + c0) + 3];  // entry of control byte c in slot 4c + 3
+      uint32_t* t = &W->aux[kTabBase + cbase + c0];  // entry of control byte c in slot kTabBase + c
       if (c0 < ctrl_n) t[0] = te.x;
-      if (c0 + 1 < ctrl_n) t[4] = te.y;
-      if (c0 + 2 < ctrl_n) t[8] = te.z;
-      if (c0 + 3 < ctrl_n) t[12] = te.w;
+      if (c0 + 1 < ctrl_n) t[1] = te.y;
+      if (c0 + 2 < ctrl_n) t[2] = te.z;
+      if (c0 + 3 < ctrl_n) t[3] = te.w;
     }
     wave_fence();
     // Lane c decodes the four values of control byte c (its table entry in
-    // slot 4c + 3) into slots 4c .. 4c+3. Slots past a block's excCount
-    // decode bytes nobody reads.
+    // slot kTabBase + c) into slots 4c .. 4c+3. Slots past a block's
+    // excCount decode bytes nobody reads.
 #pragma unroll 1
     for (int c = lane_id(); c < ctot; c += 64) {
-      const uint32_t e = W->aux[4 * c + 3];
+      const uint32_t e = W->aux[kTabBase + c];
       const uint32_t codes = e >> 24;
       const int bw = (int)((e >> kTabBwShift) & 63);
       const int o0 = (int)(e & kTabOff);
