@@ -85,7 +85,7 @@ int ctx_init(utl_ctx* c, int device) {
   HIP_TRY(hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocMapped | hipHostMallocCoherent));
   HIP_TRY(hipHostGetDevicePointer((void**)&c->d_stage, c->h_stage, 0));
   HIP_TRY(hipMalloc((void**)&c->d_counter, 4096));  // 8 per-XCD work counters, 64 B apart
-  c->tune.grid_wgs = c->cu_count * 6;  // six u32 workgroups are resident per CU
+  c->tune.grid_wgs = c->cu_count * 5;  // five persistent u32 workgroups are resident per CU
   return UTL_OK;
 }
 

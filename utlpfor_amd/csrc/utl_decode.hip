@@ -789,7 +789,7 @@ __global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 6) void k_decode(const uin
 // own work counter (counter[8x]) over XCD x's contiguous range of tiles, so
 // load balance stays inside the XCD. Every workgroup reaches the exit test.
 template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 6) void k_decode_persistent(const uint8_t* __restrict__ src,
+__global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 5) void k_decode_persistent(const uint8_t* __restrict__ src,
                                                            const uint64_t* __restrict__ boff,
                                                            const uint64_t* __restrict__ ooff, uint64_t nblocks,
                                                            void* __restrict__ out, int32_t* __restrict__ status,
