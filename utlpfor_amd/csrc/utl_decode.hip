@@ -535,10 +535,13 @@ __device__ __forceinline__ void stage_range(WL* W, const uint8_t* __restrict__ g
   wave_fence();
 }
 
-// Decoded values are written once and not read again by this kernel, so
-// they go out with non-temporal stores: they stream to HBM instead of
-// allocating L2 lines (on the headline stream this took the kernel from 98 to
-// 72 us).
+// Decoded values are written once and not read again by this kernel. A
+// full block at an even value offset writes whole 64-byte row segments, which
+// go out with non-temporal stores: they stream to HBM instead of allocating
+// L2 lines (on the headline stream this took the kernel from 98 to 72 us).
+// Every other block (partial, or at an odd offset, as in streams of arbitrary
+// lengths) writes 4-byte pieces, which must be merged into lines in L2 first:
+// non-temporal ones would go to HBM as partial lines.
 __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t out0, int count, int j,
                                           const uint32_t* v0, const uint32_t* v1) {
   uint32_t* o = out + out0 + 2 * j;
@@ -551,8 +554,8 @@ __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t o
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
       const int p = 16 * r + 2 * j;
-      if (p < count) __builtin_nontemporal_store(v0[r], o + 16 * r);
-      if (p + 1 < count) __builtin_nontemporal_store(v1[r], o + 16 * r + 1);
+      if (p < count) o[16 * r] = v0[r];
+      if (p + 1 < count) o[16 * r + 1] = v1[r];
     }
   }
 }
@@ -575,8 +578,8 @@ __device__ __forceinline__ void store_u64(uint64_t* __restrict__ out, uint64_t o
       q.y = y1;
       __builtin_nontemporal_store(q, reinterpret_cast<u64x2*>(o + 16 * r));
     } else {
-      if (p < count) __builtin_nontemporal_store(y0, o + 16 * r);
-      if (p + 1 < count) __builtin_nontemporal_store(y1, o + 16 * r + 1);
+      if (p < count) o[16 * r] = y0;
+      if (p + 1 < count) o[16 * r + 1] = y1;
     }
   }
 }
