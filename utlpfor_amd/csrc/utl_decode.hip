@@ -2,8 +2,8 @@
 //
 // Eight threads decode one block: thread j (0..7) of a block owns the UTL
 // lane pair (2j, 2j+1), i.e. the 16 values at positions 16r + 2j + e. A
-// wavefront decodes 8 consecutive blocks and a 192-thread workgroup a tile
-// of 24. Tiles are assigned XCD-major (workgroup g runs on XCD g % 8), so
+// wavefront decodes 8 consecutive blocks and a 256-thread workgroup a tile
+// of 32. Tiles are assigned XCD-major (workgroup g runs on XCD g % 8), so
 // each XCD's L2 sees one contiguous range of the stream and cache lines
 // shared by neighbouring blocks are fetched from HBM once.
 //
@@ -17,7 +17,9 @@
 //   from one flat table of control bytes; bitmap patch by popcount rank,
 //   position-list patch by LDS OR) -> per-lane delta in registers (zigzag,
 //   or overflow checked, first wrap in v-major order) -> + FOR base ->
-//   8-byte stores of the lane pairs.
+//   8-byte stores of the lane pairs, or, for rounds with blocks at odd
+//   offsets or shorter than 128 values, 16-byte stores of the round's whole
+//   output range staged in LDS.
 #include "utl_device.h"
 #include "utl_format.h"
 #include "utl_kernels.h"
@@ -560,6 +562,74 @@ __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t o
   }
 }
 
+// The values of one round whose blocks fill the output range [g0, g1)
+// without holes (every block decoded, each filling its whole slot), staged
+// in aux in output order and written out by the wave as 16-byte pieces of
+// the range: non-temporal where a piece's 64-byte segment lies inside the
+// range, cached at its two ends. Blocks at odd offsets or shorter than 128
+// values (streams of arbitrary lengths) then still write whole segments.
+// aux is free: every exception read of the round is done. act: the lane's
+// block is in the round and has count values at out0.
+template <class WL>
+__device__ __forceinline__ void store_u32_staged(WL* W, uint32_t* __restrict__ out, uint64_t g0, uint64_t g1,
+                                                 bool act, uint64_t out0, int count, int j, const uint32_t* v0,
+                                                 const uint32_t* v1) {
+  uint32_t* y = W->aux;
+  const int sh = (int)((reinterpret_cast<uintptr_t>(out + g0) >> 2) & 3);  // values before g0 in its 16 bytes
+  const int n = (int)(g1 - g0);                                            // <= 8 * 128
+  wave_fence();
+  if (act) {
+    uint32_t* z = y + sh + (int)(out0 - g0) + 2 * j;
+    if (count == kBlock) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        z[16 * r] = v0[r];
+        z[16 * r + 1] = v1[r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int p = 16 * r + 2 * j;
+        if (p < count) z[16 * r] = v0[r];
+        if (p + 1 < count) z[16 * r + 1] = v1[r];
+      }
+    }
+  }
+  wave_fence();
+  // piece k: values [4k - sh, 4k - sh + 4) of the range, staged at y[4k]
+  uint32_t* const ob = out + g0 - sh;  // 16-byte aligned
+  const int seg0 = (int)((reinterpret_cast<uintptr_t>(ob) >> 4) & 3);  // piece 0's place in its segment
+  const int npieces = (sh + n + 3) >> 2;  // <= 257: y[4k + 3] < kAux
+#pragma unroll 1
+  for (int k = lane_id(); k < npieces; k += 64) {
+    const uint4 q = *reinterpret_cast<const uint4*>(&y[4 * k]);
+    const int v = 4 * k - sh;  // first value of the piece, relative to g0
+    const int pos = (seg0 + k) & 3;  // the piece's place in its 64-byte segment
+    const int s = v - 4 * pos;       // first value of the segment
+    uint4* o = reinterpret_cast<uint4*>(ob + 4 * k);
+    // non-temporal when the whole segment is in the range and written by
+    // this one store instruction (its four pieces on lanes of one pass)
+    if (s >= 0 && s + 16 <= n && ((k - pos) & 63) <= 60) {
+      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+      u32x4 t;
+      t.x = q.x;
+      t.y = q.y;
+      t.z = q.z;
+      t.w = q.w;
+      __builtin_nontemporal_store(t, reinterpret_cast<u32x4*>(o));
+    } else if (v >= 0 && v + 4 <= n) {
+      *o = q;
+    } else {
+      uint32_t* e = ob + 4 * k;
+      if (v >= 0 && v < n) e[0] = q.x;
+      if (v + 1 >= 0 && v + 1 < n) e[1] = q.y;
+      if (v + 2 >= 0 && v + 2 < n) e[2] = q.z;
+      if (v + 3 < n) e[3] = q.w;  // v + 3 >= 0 always: v >= -3
+    }
+  }
+  wave_fence();
+}
+
 // two: values are (hi << 32) | lo; else lo + base.
 __device__ __forceinline__ void store_u64(uint64_t* __restrict__ out, uint64_t out0, int count, int j,
                                           const uint32_t* l0, const uint32_t* l1, const uint32_t* h0,
@@ -673,14 +743,27 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
       uint32_t v0[kRows], v1[kRows];
       SubResult r1;
       st = decode_sub<false>(W, active, bb, active ? at : 0, len, jj, v0, v1, r1);
+      bool put = false;
       if (active) {
         consumed = r1.consumed;
         if (st == UTL_OK && (uint64_t)r1.count > slot_n) st = UTL_ERR_ARGUMENT;
-        if (st == UTL_OK && r1.count > 0) store_u32((uint32_t*)out, out0, r1.count, jj, v0, v1);
+        put = st == UTL_OK && r1.count > 0;
         if (jj == 0) {
           status[b] = st;
           if (consumed_out) consumed_out[b] = st == UTL_OK ? consumed : 0;
         }
+      }
+      // Full blocks at even offsets write whole segments by themselves; a
+      // round with any other block goes through aux when its blocks fill
+      // one range without holes.
+      const bool direct = r1.count == kBlock && (out0 & 1) == 0;
+      const bool holes = __ballot(active && !(st == UTL_OK && (uint64_t)r1.count == slot_n)) != 0;
+      if (!holes && __ballot(put && !direct) != 0) {
+        const uint64_t g0 = readlane64(out0, kTpb * first);
+        const uint64_t g1 = readlane64(out0 + slot_n, kTpb * (last - 1));
+        store_u32_staged(W, (uint32_t*)out, g0, g1, put, out0, r1.count, jj, v0, v1);
+      } else if (put) {
+        store_u32((uint32_t*)out, out0, r1.count, jj, v0, v1);
       }
       wave_fence();
     } else {
