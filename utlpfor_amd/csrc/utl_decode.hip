@@ -431,56 +431,62 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
   // ---- per-lane delta decode (delta.go:50-115) and FOR (for.go:136-140) ----
   const uint32_t fb = (p.h.for_width > 0 && !p.u64s) ? p.for_base : 0u;
   if (p.h.delta) {
+    // Zigzag addends are decoded in place; their running sums wrap freely
+    // (delta.go:50-72). Other blocks: a lane's running sum can wrap only if
+    // its 8 addends reach 2^32, which needs one of them >= 2^29. Without
+    // such a value anywhere in the wave the prefix runs unchecked, with the
+    // base folded into row 0. (One shared prefix keeps the common case free
+    // of register copies between three paths.)
+    bool check = false;
     if (p.h.zigzag) {
-      v0[0] = zz_dec(v0[0]) + fb;
-      v1[0] = zz_dec(v1[0]) + fb;
 #pragma unroll
-      for (int r = 1; r < kRows; ++r) {
-        v0[r] = v0[r - 1] + zz_dec(v0[r]);
-        v1[r] = v1[r - 1] + zz_dec(v1[r]);
+      for (int r = 0; r < kRows; ++r) {
+        v0[r] = zz_dec(v0[r]);
+        v1[r] = zz_dec(v1[r]);
       }
     } else {
-      // A lane's running sum can wrap only if its 8 addends reach 2^32, which
-      // needs one of them >= 2^29. Without such a value anywhere in the wave
-      // the prefix runs unchecked, with the base folded into row 0.
       uint32_t big = 0;
 #pragma unroll
       for (int r = 0; r < kRows; ++r) big |= v0[r] | v1[r];
-      if (__ballot(big >= (1u << 29)) == 0) {
-        v0[0] += fb;
-        v1[0] += fb;
+      check = big >= (1u << 29);
+    }
+    if (__ballot(check) == 0) {
+      v0[0] += fb;
+      v1[0] += fb;
 #pragma unroll
-        for (int r = 1; r < kRows; ++r) {
-          v0[r] += v0[r - 1];
-          v1[r] += v1[r - 1];
-        }
-      } else {
-        uint64_t any = 0;  // lanes of the wave with a wrapped addition
+      for (int r = 1; r < kRows; ++r) {
+        v0[r] += v0[r - 1];
+        v1[r] += v1[r - 1];
+      }
+    } else {
+      uint64_t any = 0;  // lanes of the wave with a wrapped addition
 #pragma unroll
-        for (int r = 1; r < kRows; ++r) {
-          const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
-          any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
-          v0[r] = n0;
-          v1[r] = n1;
-        }
-        if (any) {
-          // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
-          // first wrap at a position < count in v-major order wins.
-          uint32_t first = 0xFFFFFFFFu;
+      for (int r = 1; r < kRows; ++r) {
+        const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
+        any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
+        v0[r] = n0;
+        v1[r] = n1;
+      }
+      if (any) {
+        // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
+        // first wrap at a position < count in v-major order wins; zigzag
+        // blocks have no overflow error.
+        uint32_t first = 0xFFFFFFFFu;
+        if (!p.h.zigzag) {
 #pragma unroll
           for (int r = kRows - 1; r >= 1; --r) {
             const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
             if (v1[r] < v1[r - 1] && (int)p1 < p.h.count) first = p1;
             if (v0[r] < v0[r - 1] && (int)p0 < p.h.count) first = p0;
           }
-          const uint32_t m = grp_min(first);
-          if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
         }
+        const uint32_t m = grp_min(first);
+        if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+      }
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-          v0[r] += fb;
-          v1[r] += fb;
-        }
+      for (int r = 0; r < kRows; ++r) {
+        v0[r] += fb;
+        v1[r] += fb;
       }
     }
   } else if (fb != 0) {
