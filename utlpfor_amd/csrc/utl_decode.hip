@@ -422,35 +422,34 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
     }
   }
 
-  if (!go) {
-    res.consumed = st == UTL_OK ? consumed : 0;  // count 0: a valid 4-byte block
-    return st;
-  }
-  if (st != UTL_OK) return st;
+  if (!go) res.consumed = st == UTL_OK ? consumed : 0;  // count 0: a valid 4-byte block
 
   // ---- per-lane delta decode (delta.go:50-115) and FOR (for.go:136-140) ----
-  const uint32_t fb = (p.h.for_width > 0 && !p.u64s) ? p.for_base : 0u;
-  if (p.h.delta) {
-    // Zigzag addends are decoded in place; their running sums wrap freely
-    // (delta.go:50-72). Other blocks: a lane's running sum can wrap only if
-    // its 8 addends reach 2^32, which needs one of them >= 2^29. Without
-    // such a value anywhere in the wave the prefix runs unchecked, with the
-    // base folded into row 0. (One shared prefix keeps the common case free
-    // of register copies between three paths.)
-    bool check = false;
-    if (p.h.zigzag) {
+  // (One exit for every path: early returns here cost register copies on
+  // the common path.)
+  if (go && st == UTL_OK) {
+    const uint32_t fb = (p.h.for_width > 0 && !p.u64s) ? p.for_base : 0u;
+    if (p.h.delta) {
+      // Zigzag addends are decoded in place; their running sums wrap freely
+      // (delta.go:50-72). The prefix runs in place for every block, with the
+      // base folded into row 0. Other blocks report the first wrap of the
+      // running sum (delta.go:83-115), which needs an addend >= 2^29 in the
+      // lane: only waves holding one look for it, on the sums less the base.
+      // (No path rewrites the values: branches that did cost register
+      // copies on the common path.)
+      bool check = false;
+      if (p.h.zigzag) {
 #pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        v0[r] = zz_dec(v0[r]);
-        v1[r] = zz_dec(v1[r]);
+        for (int r = 0; r < kRows; ++r) {
+          v0[r] = zz_dec(v0[r]);
+          v1[r] = zz_dec(v1[r]);
+        }
+      } else {
+        uint32_t big = 0;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) big |= v0[r] | v1[r];
+        check = big >= (1u << 29);
       }
-    } else {
-      uint32_t big = 0;
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) big |= v0[r] | v1[r];
-      check = big >= (1u << 29);
-    }
-    if (__ballot(check) == 0) {
       v0[0] += fb;
       v1[0] += fb;
 #pragma unroll
@@ -458,47 +457,34 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
         v0[r] += v0[r - 1];
         v1[r] += v1[r - 1];
       }
-    } else {
-      uint64_t any = 0;  // lanes of the wave with a wrapped addition
-#pragma unroll
-      for (int r = 1; r < kRows; ++r) {
-        const uint32_t n0 = v0[r - 1] + v0[r], n1 = v1[r - 1] + v1[r];
-        any |= __ballot(n0 < v0[r - 1]) | __ballot(n1 < v1[r - 1]);
-        v0[r] = n0;
-        v1[r] = n1;
-      }
-      if (any) {
-        // A wrap at row r left v[r] < v[r-1] (an addend is < 2^32). The
-        // first wrap at a position < count in v-major order wins; zigzag
-        // blocks have no overflow error.
+      if (__ballot(check) != 0) {
+        // A wrap at row r leaves sum[r] < sum[r-1] (an addend is < 2^32).
+        // The first wrap at a position < count in v-major order wins.
         uint32_t first = 0xFFFFFFFFu;
         if (!p.h.zigzag) {
 #pragma unroll
           for (int r = kRows - 1; r >= 1; --r) {
             const uint32_t p0 = 16 * r + 2 * j, p1 = p0 + 1;
-            if (v1[r] < v1[r - 1] && (int)p1 < p.h.count) first = p1;
-            if (v0[r] < v0[r - 1] && (int)p0 < p.h.count) first = p0;
+            if (v1[r] - fb < v1[r - 1] - fb && (int)p1 < p.h.count) first = p1;
+            if (v0[r] - fb < v0[r - 1] - fb && (int)p0 < p.h.count) first = p0;
           }
         }
         const uint32_t m = grp_min(first);
-        if (m != 0xFFFFFFFFu) return UTL_ERR_OVERFLOW | (int32_t)(m << 8);
+        if (m != 0xFFFFFFFFu) st = UTL_ERR_OVERFLOW | (int32_t)(m << 8);
       }
+    } else if (fb != 0) {
 #pragma unroll
       for (int r = 0; r < kRows; ++r) {
         v0[r] += fb;
         v1[r] += fb;
       }
     }
-  } else if (fb != 0) {
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      v0[r] += fb;
-      v1[r] += fb;
+    if (st == UTL_OK) {
+      res.count = p.h.count;
+      res.consumed = consumed;
     }
   }
-  res.count = p.h.count;
-  res.consumed = consumed;
-  return UTL_OK;
+  return st;
 }
 
 // Staged extent of a block: u32 blocks read at most kSubExtent bytes; a
