@@ -330,8 +330,9 @@ def test_batched_mixed_streams_and_corruption(gpu, orc):
 
 def test_uneven_streams_graph_autotune(gpu, orc):
     """Mixed-flag uneven streams: GPU encode bytes equal the oracle's stream
-    by stream; the autotuned persistent decode captured as a HIP graph equals
-    the flat decode on every replay (eager work and syncs between replays)."""
+    by stream; the autotuned persistent decode and the flat decode, each
+    captured with the index scan as a HIP graph, equal the eager flat decode
+    on every replay (eager work and syncs between replays)."""
     from utlpfor_amd import autotune, batch, workloads
     from utlpfor_amd.device import DeviceArray
     codec, ctx = gpu
@@ -355,18 +356,19 @@ def test_uneven_streams_graph_autotune(gpu, orc):
     assert tune["grid_wgs"] % 8 == 0 and tune["tiles_per_grab"] >= 1
     out = DeviceArray(v.size, np.uint32, ctx)
     st = DeviceArray(s.nblocks, np.int32, ctx)
-    g = batch.DecodeGraph(ctx, s, out, st, persistent=True)
-    for rep in range(3):
-        out.fill_bytes(0xA5)
-        ctx.sync()
-        g.launch()
-        ctx.sync()
-        assert np.all(st.to_numpy() == 0), rep
-        assert np.array_equal(out.to_numpy(), ref), rep
-        # eager work between replays
-        batch.decode(ctx, s, out=flat_out, status=flat_st)
-        ctx.sync()
-    g.close()
+    for persistent in (True, False):
+        g = batch.DecodeGraph(ctx, s, out, st, persistent=persistent)
+        for rep in range(3):
+            out.fill_bytes(0xA5)
+            ctx.sync()
+            g.launch()
+            ctx.sync()
+            assert np.all(st.to_numpy() == 0), (persistent, rep)
+            assert np.array_equal(out.to_numpy(), ref), (persistent, rep)
+            # eager work between replays
+            batch.decode(ctx, s, out=flat_out, status=flat_st)
+            ctx.sync()
+        g.close()
     s.free()
 
 

@@ -108,7 +108,9 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
     st = DeviceArray(s.nblocks, np.int32, ctx)
     say(f"{s.nblocks} blocks, {s.nbytes} bytes; autotuning the persistent decode")
     tune = autotune.tune_decode(ctx, s, out, st, use_cache=False, reps=3)
-    graph = batch.DecodeGraph(ctx, s, out, st, persistent=True)
+    # the graph holds whichever decode form the tuner measured faster
+    persistent = tune["ms"] < tune["flat_ms"]
+    graph = batch.DecodeGraph(ctx, s, out, st, persistent=persistent)
     ms = _time(ctx, graph.launch, reps)
     flat_ms = _time(ctx, lambda: batch.decode(ctx, s, out=out, status=st), reps)
     graph.launch()
@@ -125,7 +127,8 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
     say("foreign-stream index walk")
     wrec = walk(ctx, s, s.block_off.to_numpy()[batch.plan_blocks(lens).astype(np.int64)])
     rec = _decode_record(f"{nstreams} uneven streams, lengths logU[1,200000], mixed flags", s, ms, v.size, 4,
-                         {"bit_exact": ok, "form": "persistent decode + index scan in one hipGraph launch",
+                         {"bit_exact": ok,
+                          "form": f"{'persistent' if persistent else 'flat'} decode + index scan in one hipGraph launch",
                           "flat_decode_ms": round(flat_ms, 5),
                           "tune": {k: tune[k] for k in ("grid_wgs", "tiles_per_grab", "ms", "flat_ms")},
                           "streams": nstreams, "gen_seconds": round(gen_s, 1), "walk": wrec})
