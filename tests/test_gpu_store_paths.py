@@ -1,7 +1,8 @@
 """Where the batched u32 decode writes: streams of arbitrary lengths (blocks
 at odd value offsets, partial blocks) decoded into output buffers at every
 16-byte phase, by the flat and the persistent kernel; failed blocks inside
-otherwise whole rounds; and output slots wider than their blocks. Values
+otherwise whole rounds; and output slots wider than their blocks (decode,
+and Get by global position over that index). Values
 outside the decoded slots, and the slots of failed blocks, keep the bytes
 the buffer held before (the C-ABI's contract: values of failed blocks are
 not written)."""
@@ -144,4 +145,17 @@ def test_slots_wider_than_blocks(ctx):
         a, c = int(wide[k]), int(counts[k])
         assert np.array_equal(got[a:a + c], v[int(ooff[k]):int(ooff[k]) + c]), k
         assert np.all(got[a + c:int(wide[k + 1])] == POISON), k
+    # Get by global position over the same index: a position in a block's
+    # slot reads its value; one in the gap after it is past that block's
+    # count (PositionOutOfRange). The block search starts at pos / 128,
+    # which overshoots here and must walk back.
+    pos = rng.integers(0, int(wide[-1]), 20000).astype(np.int64)
+    blk = np.searchsorted(wide, pos, side="right") - 1
+    inb = pos - wide[blk].astype(np.int64)
+    valid = inb < counts[blk].astype(np.int64)
+    vals, gst = batch.get_batch(ctx, s2, DeviceArray.from_numpy(pos, ctx))
+    ctx.sync()
+    gv, gs = vals.to_numpy(), gst.to_numpy()
+    assert np.all(gs[valid] == 0) and np.all(gs[~valid] == 2)  # ErrPositionOutOfRange
+    assert np.array_equal(gv[valid], v[(ooff[blk] + inb)[valid].astype(np.int64)])
     s.free()

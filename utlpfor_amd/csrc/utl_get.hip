@@ -355,7 +355,33 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
     st = b < nblocks ? UTL_OK : UTL_ERR_ARGUMENT;
   } else {
     st = (pos >= 0 && (uint64_t)pos < ooff[nblocks]) ? UTL_OK : UTL_ERR_POSITION_OUT_OF_RANGE;
-    uint64_t lo = 0, hi = nblocks;  // largest b with ooff[b] <= pos
+    // Largest b with ooff[b] <= pos. Blocks of an index built from meta
+    // words hold at most 128 values, so b >= pos / 128, and b == pos / 128
+    // when the blocks before are full: gallop from there to a bracket
+    // [lo, hi) with ooff[lo] <= pos < ooff[hi] (hi == nblocks: past the
+    // end), then bisect. Any nondecreasing out_off gives the same b as a
+    // bisection of the whole array.
+    uint64_t lo = 0, hi = nblocks;
+    const uint64_t g = st == UTL_OK ? min((uint64_t)pos >> 7, nblocks - 1) : 0;
+    if (ooff[g] <= (uint64_t)pos) {
+      lo = g;
+      for (uint64_t step = 1; lo + step < nblocks; step <<= 1) {
+        if (ooff[lo + step] > (uint64_t)pos) {
+          hi = lo + step;
+          break;
+        }
+        lo += step;
+      }
+    } else {
+      hi = g;
+      for (uint64_t step = 1; step <= hi; step <<= 1) {
+        if (ooff[hi - step] <= (uint64_t)pos) {
+          lo = hi - step;
+          break;
+        }
+        hi -= step;
+      }
+    }
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
       if (ooff[mid] <= (uint64_t)pos) lo = mid;
