@@ -34,10 +34,12 @@ constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 // block2Len (16) + payload (512) + bitmap (16) + control (32) + data (512).
 constexpr int kSubExtent = 1088;
 // Staged bytes per wavefront: u32 blocks use 2560 (>= 15 + 2176 + 4); u64
-// blocks 4608, which holds four two-block blocks of bw 32 per round and
-// costs no workgroup per CU (that kernel is held to 4 by its registers).
+// blocks 8320, which holds all eight blocks of a wave in one round even when
+// they are two-block blocks of bw 32 (7 * 1034 + 1034 + 19 bytes), at three
+// workgroups per CU (one fewer than the registers allow; two staging round
+// trips per wave cost more: 8Mi uniform two-block 30.5 -> 24.7 us).
 constexpr int kArena32 = 2560;
-constexpr int kArena64 = 4608;
+constexpr int kArena64 = 8320;
 // Exceptions of the wave's blocks are numbered flat: block b's control bytes
 // are control bytes [cbase_b, cbase_b + ctrl_b) of the wave, with cbase_b the
 // control bytes of the blocks before it, and its exception i is slot
@@ -53,10 +55,10 @@ constexpr int kCtrlMax = kWaveBlocks * 32;       // control bytes of a wave's bl
 constexpr int kTabBase = kWaveBlocks * kBlock - kCtrlMax;  // 768
 static_assert(4 * 64 * (kCtrlMax / 64 - 1) + 4 * 64 <= kTabBase + kCtrlMax, "table rounds");
 
-// A table entry holds a data offset into the arena (13 bits: every arena is
-// below 8 KiB), the block's bit width at bit 13 and the control byte at 24.
-constexpr uint32_t kTabOff = 0x1FFF;
-constexpr int kTabBwShift = 13;
+// A table entry holds a data offset into the arena (14 bits: every arena is
+// below 16 KiB), the block's bit width at bit 14 and the control byte at 24.
+constexpr uint32_t kTabOff = 0x3FFF;
+constexpr int kTabBwShift = 14;
 
 template <int ARENA>
 struct alignas(16) WaveLdsT {
@@ -65,11 +67,11 @@ struct alignas(16) WaveLdsT {
   uint32_t arena[ARENA / 4];
   uint32_t aux[kAux];  // exception high bits by flat slot (and the table entries before that)
 };
-// u32: 4 x 6672 B per workgroup, 6 workgroups per CU; u64: 4 x 8720 B, 4
-// workgroups per CU (that kernel is held to 4 by its registers).
+// u32: 4 x 6672 B per workgroup, 6 workgroups per CU; u64: 4 x 12432 B, 3
+// workgroups per CU.
 static_assert(sizeof(WaveLdsT<kArena32>) == 6672, "LDS per wavefront");
 static_assert(6 * 4 * sizeof(WaveLdsT<kArena32>) + 8 <= 160 * 1024, "six u32 workgroups per CU");
-static_assert(4 * 4 * sizeof(WaveLdsT<kArena64>) + 8 <= 160 * 1024, "four u64 workgroups per CU");
+static_assert(3 * 4 * sizeof(WaveLdsT<kArena64>) + 8 <= 160 * 1024, "three u64 workgroups per CU");
 template <bool U64>
 struct WaveLdsSel {
   using type = WaveLdsT<kArena32>;
@@ -700,29 +702,34 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
   int first = 0;
 #pragma unroll 1
   while (first < nb) {
-    uint64_t base, lo;
+    uint64_t base;
     int last;
-    int64_t ext_last;
+    int64_t span;  // staged bytes from base
     if (one_round) {
       base = o_first;
       last = nb;
-      lo = o_lastm1;
-      ext_last = ext_all;
+      span = (int64_t)(o_lastm1 - o_first) + ext_all;
     } else {
+      // Blocks from `first` on while each one's extent fits the arena. Their
+      // starts need not increase (a gathered index jumps between requests),
+      // so the round stages up to the furthest extent among them, not the
+      // last one's.
       base = boff[b0 + first];
       last = first;
+      span = 0;
 #pragma unroll 1
       while (last < nb) {
         const uint64_t o0 = boff[b0 + last], o1 = boff[b0 + last + 1];
         const int64_t ext = clamp_extent<U64>((int64_t)(o1 - o0));
         if (o0 < base || (int64_t)(o0 - base) + ext + 15 + 4 > kArena) break;
+        span = max(span, (int64_t)(o0 - base) + ext);
         ++last;
       }
-      if (last == first) last = first + 1;
-      lo = boff[b0 + last - 1];
-      ext_last = clamp_extent<U64>((int64_t)(boff[b0 + last] - lo));
+      if (last == first) {  // (block `first` always fits: its extent is at most kExtent64)
+        last = first + 1;
+        span = clamp_extent<U64>((int64_t)(boff[b0 + first + 1] - base));
+      }
     }
-    const int64_t span = (last - first == 1) ? ext_last : (int64_t)(lo - base) + ext_last;
     const uint8_t* gbase = src + base;
     stage_range(W, gbase, span);
     int jj = j, bb = blk;  // opaque per round: keeps per-thread LDS addresses out of the round loop
