@@ -271,10 +271,18 @@ template <bool ForU64, class WL>
 __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, int64_t len, int j,
                                               uint32_t* v0, uint32_t* v1, SubResult& res) {
   const uint32_t* a = W->arena;
-  SubPlan p{};  // fields the prelude does not reach stay 0
-  bool go = false;
-  int consumed = 0;
-  int32_t st = act ? sub_prelude<ForU64>(a, at, len, p, go, consumed) : UTL_OK;
+  // Every lane parses a header (lanes not in the round read the arena's
+  // first bytes: callers pass at = 0), so there is no branch and no default
+  // plan to set up; those lanes then take part in nothing.
+  SubPlan p;
+  bool go;
+  int consumed;
+  int32_t st = sub_prelude<ForU64>(a, at, len, p, go, consumed);
+  if (!act) {
+    st = UTL_OK;
+    go = false;
+    consumed = 0;
+  }
   res.count = 0;
   res.consumed = 0;
   if (go) unpack_any(p.h.bw, a, at + p.poff, j, v0, v1);
