@@ -171,7 +171,10 @@ int utl_index_walk(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_stream_
 
 /* Batched UnpackUint32 over an index (one launch). d_status[k] receives the
  * block's status word; d_consumed (optional, may be NULL) its consumed
- * bytes. Values of failed blocks are not written. */
+ * bytes. Block k's values go to d_out[d_out_off[k] ..]; values of failed
+ * blocks are not written, nor is any slot position past a block's count.
+ * The index may jump between regions of d_src (a gathered index). d_out is
+ * naturally aligned. */
 int utl_decode_u32(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_off, const uint64_t *d_out_off,
                    size_t nblocks, uint32_t *d_out, int32_t *d_status, int32_t *d_consumed);
 
@@ -182,7 +185,10 @@ int utl_decode_u64(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_o
 /* Persistent form of the batched decode (is_u64 selects UnpackUint64): a
  * fixed grid of wavefronts pulls runs of blocks from a device work queue.
  * Launch shape from utl_set_tune (chosen by the autotuner in
- * utlpfor_amd/autotune.py); results identical to utl_decode_u32/u64. */
+ * utlpfor_amd/autotune.py); results identical to utl_decode_u32/u64. On
+ * gfx950 it is slower than the flat launch (a looping wavefront waits for
+ * its previous stores with every load): use it where a fixed grid is
+ * wanted. */
 int utl_decode_persistent(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
                           const uint64_t *d_out_off, size_t nblocks, void *d_out, int32_t *d_status,
                           int32_t *d_consumed);
@@ -191,9 +197,10 @@ int utl_get_tune(utl_ctx *ctx, int *waves_per_wg, int *grid_wgs, int *blocks_per
 
 /* Batched GetUint32 / GetUint64 (get.go:74, uint64.go:364): request i reads
  * position d_req_pos[i] of block d_req_block[i]; with d_req_block NULL,
- * d_req_pos[i] is a position in the decoded stream and its block is found by
- * binary search over d_out_off. d_out receives 4- or 8-byte values,
- * d_status the per-request status word. */
+ * d_req_pos[i] is a position in the decoded stream and its block, the last
+ * one whose d_out_off entry is <= the position, is found by a search that
+ * starts at position / 128 (d_out_off nondecreasing). d_out receives 4- or
+ * 8-byte values, d_status the per-request status word. */
 int utl_get_batch(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
                   const uint64_t *d_out_off, size_t nblocks, const uint64_t *d_req_block, const int64_t *d_req_pos,
                   size_t nreq, void *d_out, int32_t *d_status);
