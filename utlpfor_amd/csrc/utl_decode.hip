@@ -372,9 +372,18 @@ __device__ __forceinline__ int32_t decode_sub(WL* W, bool act, int blk, int at, 
     } else {
       // 128-bit bitmap masked to count; the rank of a set bit is the index
       // of its high bits. popcount(bitmap) > excCount applies nothing.
+      // the 16 bitmap bytes: the five dwords that hold them, read from one
+      // address, aligned in registers
       uint32_t bm[4];
+      {
+        const uint32_t* q = a + (exc_start >> 2);
+        const int sh = exc_start & 3;
+        uint32_t w[5];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) bm[d] = lds_u32(a, exc_start + 4 * d);
+        for (int d = 0; d < 5; ++d) w[d] = q[d];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) bm[d] = __builtin_amdgcn_alignbyte(w[d + 1], w[d], sh);
+      }
       if (p.h.count < kBlock) {
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
