@@ -118,6 +118,11 @@ int utl_get_u64(utl_ctx *ctx, int64_t pos, const uint8_t *src, size_t len, uint6
 /* ---- device memory (the batched forms work on device-resident data) ---- */
 int utl_dev_alloc(void **ptr, size_t bytes);
 int utl_dev_free(void *ptr);
+/* Pinned (page-locked) host memory: copies to and from it run at the full
+ * link rate. No counterpart in the reference (Go owns host memory); used by
+ * the serving loop to deliver results. */
+int utl_host_alloc(void **ptr, size_t bytes);
+int utl_host_free(void *ptr);
 int utl_memcpy_h2d(utl_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
 int utl_memcpy_d2h(utl_ctx *ctx, void *dst, const void *src, size_t bytes);  /* synchronous */
 int utl_memset_d(utl_ctx *ctx, void *dst, int value, size_t bytes);          /* fill kernel, async */

@@ -114,3 +114,30 @@ def test_serving_admission_by_heap(ctx):
     for r, w in zip(reqs, want):
         assert np.array_equal(r.result, w)
     store.free()
+
+
+def test_host_stage_chunks_and_threads(ctx):
+    """Result delivery through the pinned staging buffer: a buffer much
+    smaller than the range (many chunks, destinations straddling chunk
+    ends) and pieces small enough that the thread pool copies most of them.
+    Every destination receives exactly its device bytes."""
+    from utlpfor_amd import serve
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(71)
+    n = 3_000_001
+    src = rng.integers(0, 256, n, dtype=np.uint8)
+    d = DeviceArray.from_numpy(src, ctx)
+    stage = serve._HostStage(ctx, nbytes=100_003, threads=4)
+    stage.piece = 4096
+    cuts = np.sort(rng.choice(np.arange(1, n), 500, replace=False))
+    edges = np.concatenate([[0], cuts, [n]])
+    dsts = []
+    for a, b in zip(edges[:-1], edges[1:]):
+        if rng.random() < 0.8:  # leave some gaps
+            dt = np.uint32 if (b - a) % 4 == 0 and rng.random() < 0.5 else np.uint8
+            dsts.append((int(a), np.empty((b - a) // np.dtype(dt).itemsize, dt)))
+    ctx.sync()
+    stage.fetch(d.ptr, n, list(reversed(dsts)))
+    for off, arr in dsts:
+        assert np.array_equal(arr.view(np.uint8), src[off:off + arr.nbytes]), off
+    stage.close()

@@ -39,6 +39,8 @@ SIGNATURES = {
     "utl_get_u64": (C.c_int, [vp, C.c_int64, u8p, sz, u64p]),
     "utl_dev_alloc": (C.c_int, [C.POINTER(vp), sz]),
     "utl_dev_free": (C.c_int, [vp]),
+    "utl_host_alloc": (C.c_int, [C.POINTER(vp), sz]),
+    "utl_host_free": (C.c_int, [vp]),
     "utl_memcpy_h2d": (C.c_int, [vp, vp, vp, sz]),
     "utl_memcpy_d2h": (C.c_int, [vp, vp, vp, sz]),
     "utl_memset_d": (C.c_int, [vp, vp, C.c_int, sz]),

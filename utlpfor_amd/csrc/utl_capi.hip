@@ -384,6 +384,20 @@ int utl_dev_free(void* ptr) {
   return UTL_OK;
 }
 
+int utl_host_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return UTL_ERR_ARGUMENT;
+  *ptr = nullptr;
+  int err;
+  if (!resolve(nullptr, &err)) return err;
+  HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return UTL_OK;
+}
+
+int utl_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return UTL_OK;
+}
+
 int utl_memcpy_h2d(utl_ctx* ctx, void* dst, const void* src, size_t bytes) {
   int err;
   utl_ctx* c = resolve(ctx, &err);

@@ -143,6 +143,32 @@ def dev_copy(ctx: Context, dst: DeviceArray, src: DeviceArray, nbytes: int) -> N
     check(_lib.load().utl_dev_copy(ctx.handle, dst.ptr, src.ptr, nbytes), "utl_dev_copy")
 
 
+class PinnedBuffer:
+    """Page-locked host memory (utl_host_alloc): device copies into it run
+    at the full link rate (on MI355X about 57 GB/s against 10-17 GB/s into
+    pageable memory)."""
+
+    def __init__(self, nbytes: int):
+        self.lib = _lib.load()
+        p = C.c_void_p()
+        check(self.lib.utl_host_alloc(C.byref(p), max(int(nbytes), 1)), "utl_host_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        self.bytes = np.ctypeslib.as_array((C.c_uint8 * max(self.nbytes, 1)).from_address(self.ptr))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.bytes = None
+            self.lib.utl_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def mem_info():
     lib = _lib.load()
     f, t = C.c_size_t(), C.c_size_t()

@@ -169,7 +169,7 @@ def serving(ctx: Context, store_bytes: int = 16 << 30, rate: float = 2000.0, nre
         if deliver:
             stats["bit_exact_sampled"] = _check_served(store, reqs[::check_every])
         out["to_host" if deliver else "in_hbm"] = stats
-        eng.heap.mem.free()
+        eng.close()
         if progress:
             progress(f"served ({'to host' if deliver else 'in HBM'}): {stats}")
     store.free()

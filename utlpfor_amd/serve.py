@@ -27,12 +27,13 @@ from __future__ import annotations
 import ctypes as C
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _lib, batch
-from .device import Context, DeviceArray, check, mem_info
+from .device import Context, DeviceArray, PinnedBuffer, check, mem_info
 
 DECOMPRESS, COMPRESS, GET = 0, 1, 2
 KIND_NAMES = ("decompress", "compress", "get")
@@ -133,6 +134,52 @@ class _Heap:
         return self.mem.ptr + off
 
 
+class _HostStage:
+    """Delivery of results to the host. A device range comes over in chunks
+    of a pinned buffer (full link rate), and each chunk is copied out to the
+    result arrays it holds by a pool of threads (numpy copies release the
+    GIL; one host thread copies at about 14 GB/s)."""
+
+    def __init__(self, ctx: Context, nbytes: int = 256 << 20, threads: int = 8):
+        self.ctx, self.lib = ctx, _lib.load()
+        self.pin = PinnedBuffer(nbytes)
+        self.pool = ThreadPoolExecutor(max_workers=threads)
+        self.piece = 4 << 20  # bytes one thread copies at a time
+
+    def fetch(self, dev: int, nbytes: int, dsts: list[tuple[int, np.ndarray]]) -> None:
+        """Copies device bytes [dev + off, dev + off + a.nbytes) into array a
+        for every (off, a) in dsts (contiguous arrays, disjoint ranges inside
+        [0, nbytes))."""
+        dsts = sorted(((off, a.reshape(-1).view(np.uint8)) for off, a in dsts if a.nbytes), key=lambda t: t[0])
+        cap = self.pin.nbytes
+        c0, k = 0, 0
+        while c0 < nbytes and k < len(dsts):
+            c1 = min(c0 + cap, nbytes)
+            check(self.lib.utl_memcpy_d2h(self.ctx.handle, self.pin.ptr, dev + c0, c1 - c0), "d2h")
+            jobs = []
+            j = k
+            while j < len(dsts) and dsts[j][0] < c1:
+                off, a = dsts[j]
+                lo, hi = max(off, c0), min(off + a.nbytes, c1)
+                for x in range(lo, hi, self.piece):
+                    y = min(x + self.piece, hi)
+                    src, dst = self.pin.bytes[x - c0:y - c0], a[x - off:y - off]
+                    if hi - lo <= self.piece:
+                        dst[:] = src
+                    else:
+                        jobs.append(self.pool.submit(np.copyto, dst, src))
+                if off + a.nbytes <= c1:
+                    k = j + 1
+                j += 1
+            for f in jobs:
+                f.result()
+            c0 = c1
+
+    def close(self) -> None:
+        self.pool.shutdown()
+        self.pin.free()
+
+
 def _values_of(st: StoreStream, first: int, nblocks: int) -> int:
     return max(0, min(nblocks * 128, st.nvalues - first * 128))
 
@@ -147,6 +194,12 @@ class Engine:
         self.ctx, self.store = ctx, store
         self.lib = _lib.load()
         self.heap = _Heap(ctx, heap_bytes)
+        self.stage = _HostStage(ctx)
+
+    def close(self) -> None:
+        """Frees the heap and the host staging buffer."""
+        self.stage.close()
+        self.heap.mem.free()
 
     # ---- admission ----
     def cost(self, r: Request) -> int:
@@ -236,11 +289,12 @@ class Engine:
         check(f(ctx.handle, store.data.ptr, gboff, gooff, e - 1, out, status, None), "gathered decode")
 
         def deliver():
-            vals = self._d2h(out, u, np.uint64 if is64 else np.uint32)
-            sts = self._d2h(status, max(e - 1, 0), np.int32)
+            dt = np.uint64 if is64 else np.uint32
             for r, (u0, nv, e0, nb) in zip(reqs, spans):
-                r.result = vals[u0:u0 + nv].copy()
-                r.status = sts[e0:e0 + nb].copy()
+                r.result = np.empty(nv, dt)
+                r.status = np.empty(nb, np.int32)
+            self.stage.fetch(out, u * item, [(u0 * item, r.result) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
+            self.stage.fetch(status, 4 * max(e - 1, 0), [(4 * e0, r.status) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
         return [deliver]
 
     def _get(self, reqs: list[Request], is64: bool):
@@ -262,11 +316,12 @@ class Engine:
                                 store.nblocks, d_blk, d_pos, n, out, status), "utl_get_batch")
 
         def deliver():
-            vals = self._d2h(out, n, np.uint64 if is64 else np.uint32)
-            sts = self._d2h(status, n, np.int32)
+            dt, item = (np.uint64, 8) if is64 else (np.uint32, 4)
             for r, (a, k) in zip(reqs, spans):
-                r.result = vals[a:a + k].copy()
-                r.status = sts[a:a + k].copy()
+                r.result = np.empty(k, dt)
+                r.status = np.empty(k, np.int32)
+            self.stage.fetch(out, item * n, [(item * a, r.result) for r, (a, k) in zip(reqs, spans)])
+            self.stage.fetch(status, 4 * n, [(4 * a, r.status) for r, (a, k) in zip(reqs, spans)])
         return [deliver]
 
     def _compress(self, reqs: list[Request], is64: bool):
@@ -298,11 +353,13 @@ class Engine:
         def deliver():
             bo = self._d2h(boff, nb + 1, np.uint64)
             metas = self._d2h(meta, nb, np.uint32)
-            allb = self._d2h(data, int(bo[-1]), np.uint8)
+            dsts = []
             for i, r in enumerate(reqs):
                 b0, b1 = int(blk_off[i]), int(blk_off[i + 1])
-                r.result = allb[int(bo[b0]):int(bo[b1])].copy()
+                r.result = np.empty(int(bo[b1]) - int(bo[b0]), np.uint8)
                 r.block_len = (metas[b0:b1] & 0xFFFFFF).astype(np.int64)
+                dsts.append((int(bo[b0]), r.result))
+            self.stage.fetch(data, int(bo[-1]), dsts)
         return [deliver]
 
     # ---- the loop ----
