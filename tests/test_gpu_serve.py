@@ -141,3 +141,25 @@ def test_host_stage_chunks_and_threads(ctx):
     for off, arr in dsts:
         assert np.array_equal(arr.view(np.uint8), src[off:off + arr.nbytes]), off
     stage.close()
+
+
+def test_host_stage_put(ctx):
+    """Uploads through the pinned staging buffer: arrays of several dtypes
+    at their offsets, across many small chunks, land byte for byte."""
+    from utlpfor_amd import serve
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(72)
+    n = 2_000_003
+    want = rng.integers(0, 256, n, dtype=np.uint8)
+    edges = np.concatenate([[0], np.sort(rng.choice(np.arange(1, n), 300, replace=False)), [n]])
+    srcs = []
+    for a, b in zip(edges[:-1], edges[1:]):
+        piece = want[a:b]
+        srcs.append((int(a), piece.view(np.uint32).copy() if (b - a) % 4 == 0 else piece.copy()))
+    d = DeviceArray(n, np.uint8, ctx)
+    stage = serve._HostStage(ctx, nbytes=65_537, threads=4)
+    stage.piece = 2048
+    stage.put(d.ptr, n, srcs[::-1])
+    ctx.sync()
+    assert np.array_equal(d.to_numpy(), want)
+    stage.close()

@@ -175,6 +175,35 @@ class _HostStage:
                 f.result()
             c0 = c1
 
+    def put(self, dev: int, nbytes: int, srcs: list[tuple[int, np.ndarray]]) -> None:
+        """The other direction: array a's bytes to device [dev + off, ...)
+        for every (off, a) in srcs (contiguous arrays, disjoint ranges that
+        cover [0, nbytes)), through the pinned buffer chunk by chunk."""
+        srcs = sorted(((off, a.reshape(-1).view(np.uint8)) for off, a in srcs if a.nbytes), key=lambda t: t[0])
+        cap = self.pin.nbytes
+        c0, k = 0, 0
+        while c0 < nbytes:
+            c1 = min(c0 + cap, nbytes)
+            jobs = []
+            j = k
+            while j < len(srcs) and srcs[j][0] < c1:
+                off, a = srcs[j]
+                lo, hi = max(off, c0), min(off + a.nbytes, c1)
+                for x in range(lo, hi, self.piece):
+                    y = min(x + self.piece, hi)
+                    dst, src = self.pin.bytes[x - c0:y - c0], a[x - off:y - off]
+                    if hi - lo <= self.piece:
+                        dst[:] = src
+                    else:
+                        jobs.append(self.pool.submit(np.copyto, dst, src))
+                if off + a.nbytes <= c1:
+                    k = j + 1
+                j += 1
+            for f in jobs:
+                f.result()
+            check(self.lib.utl_memcpy_h2d(self.ctx.handle, dev + c0, self.pin.ptr, c1 - c0), "h2d")
+            c0 = c1
+
     def close(self) -> None:
         self.pool.shutdown()
         self.pin.free()
@@ -331,7 +360,10 @@ class Engine:
         blk_off = batch.plan_blocks(lens)
         nb = int(blk_off[-1])
         val_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
-        d_vals = self._h2d(np.concatenate([r.values.astype(dtype, copy=False) for r in reqs]))
+        item = np.dtype(dtype).itemsize
+        d_vals = self.heap.take(max(int(val_off[-1]) * item, 1))
+        self.stage.put(d_vals, int(val_off[-1]) * item,
+                       [(int(val_off[i]) * item, np.ascontiguousarray(r.values, dtype=dtype)) for i, r in enumerate(reqs)])
         d_val_off = self._h2d(val_off)
         d_blk_off = self._h2d(blk_off)
         d_flags = self._h2d(np.array([r.flag for r in reqs], dtype=np.uint8))
