@@ -1,0 +1,132 @@
+"""Randomised differential test of the batched decode against the oracle.
+
+Each case builds a store of streams (random flags, lengths and value
+distributions, u32 or u64), damages a share of its blocks byte by byte,
+and decodes random block ranges of it gathered into one index
+(utl_gather_index): ranges overlap, repeat and go backwards, and the bridge
+entries between them span arbitrary byte ranges, backwards too. The flat
+and the persistent decode run over that index into a poisoned output.
+
+The oracle decodes the same blocks from a copy buffer that holds each
+gathered block's own bytes one after another (Unpack reads nothing outside
+its block, so the bytes around it do not matter); a backward range is an
+empty block there, which Unpack rejects with ErrInvalidBuffer as the
+product does. Status words, overflow positions and the values of decoded
+blocks must be identical, and slots of failed blocks keep the poison.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+POISON = 0xA5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from utlpfor_amd.device import Context
+    return Context(0)
+
+
+def _values(rng, n, is64):
+    k = int(rng.integers(0, 5))
+    top = 64 if is64 else 32
+    if k == 0:
+        v = rng.integers(0, 1 << int(rng.integers(0, top)), n, dtype=np.uint64)
+    elif k == 1:  # sorted with big jumps (delta, overflow near the top)
+        v = np.cumsum(rng.integers(0, 1 << int(rng.integers(1, 28)), n, dtype=np.uint64))
+    elif k == 2:  # small values with outliers (exceptions)
+        v = rng.integers(0, 1 << 10, n, dtype=np.uint64)
+        m = rng.random(n) < rng.uniform(0.01, 0.4)
+        v[m] = rng.integers(1 << 20, 1 << (top - 1), int(m.sum()), dtype=np.uint64)
+    elif k == 3:  # random walk (zigzag under delta)
+        v = (np.cumsum(rng.integers(-5000, 5001, n)) + (1 << 20)).astype(np.int64).astype(np.uint64)
+    else:
+        v = rng.integers(0, 1 << 63, n, dtype=np.uint64) * 2 + rng.integers(0, 2, n, dtype=np.uint64)
+    return v if is64 else (v & 0xFFFFFFFF).astype(np.uint32)
+
+
+def _case(ctx, orc, seed, is64, persistent):
+    from utlpfor_amd import _lib, batch
+    from utlpfor_amd.device import DeviceArray, check
+    lib = _lib.load()
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0, np.log(6000), int(rng.integers(20, 60))))).astype(np.int64)
+    flags = rng.choice([0, 1, 2, 3, 4, 5, 6, 7], lens.size).astype(np.uint8)
+    v = np.concatenate([_values(rng, int(L), is64) for L in lens])
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    data = s.bytes_host().copy()
+    boff, meta = s.block_off.to_numpy(), s.meta.to_numpy()
+    # damage: a few random bytes of a random share of the blocks
+    share = rng.uniform(0.0, 0.3)
+    for k in np.nonzero(rng.random(s.nblocks) < share)[0]:
+        a, b = int(boff[k]), int(boff[k + 1])
+        for _ in range(int(rng.integers(1, 4))):
+            data[int(rng.integers(a, b))] = int(rng.integers(0, 256))
+    cs = batch.from_host_blocks(ctx, data, meta & 0xFFFFFF, meta >> 24, is64=is64)
+    ooff = cs.out_off.to_numpy()
+    # requests: random ranges, overlapping, repeated, stepping back
+    rows, u, e = [], 0, 0
+    for _ in range(int(rng.integers(50, 300))):
+        f = int(rng.integers(0, cs.nblocks))
+        c = int(rng.integers(1, min(20, cs.nblocks - f) + 1))
+        rows.append((f, c, u, e))
+        u += int(ooff[f + c] - ooff[f])
+        e += c + 1
+    req = np.array(rows, np.uint64)
+    gboff, gooff = DeviceArray(e, np.uint64, ctx), DeviceArray(e, np.uint64, ctx)
+    check(lib.utl_gather_index(ctx.handle, cs.block_off.ptr, cs.out_off.ptr, cs.nblocks,
+                               DeviceArray.from_numpy(req.reshape(-1), ctx).ptr, len(rows), e, gboff.ptr,
+                               gooff.ptr), "utl_gather_index")
+    ng = e - 1
+    item = 8 if is64 else 4
+    out = DeviceArray(max(u, 1) * item, np.uint8, ctx)
+    out.fill_bytes(POISON)
+    st = DeviceArray(ng, np.int32, ctx)
+    ctx.sync()
+    say(f"fuzz seed {seed} {'u64' if is64 else 'u32'} persistent={persistent}: {lens.size} streams, "
+        f"{cs.nblocks} blocks ({share:.0%} damaged), {len(rows)} requests, {ng} gathered blocks")
+    if persistent:
+        code = lib.utl_decode_persistent(ctx.handle, int(is64), cs.data.ptr, gboff.ptr, gooff.ptr, ng, out.ptr,
+                                         st.ptr, None)
+    else:
+        f = lib.utl_decode_u64 if is64 else lib.utl_decode_u32
+        code = f(ctx.handle, cs.data.ptr, gboff.ptr, gooff.ptr, ng, out.ptr, st.ptr, None)
+    check(code, "decode")
+    ctx.sync()
+    gb, go = gboff.to_numpy(), gooff.to_numpy()
+    # the oracle's copy buffer: each gathered block's own bytes
+    lens_g = np.where(gb[1:] >= gb[:-1], gb[1:] - gb[:-1], 0).astype(np.int64)
+    coff = np.concatenate([[0], np.cumsum(lens_g)]).astype(np.uint64)
+    cbuf = np.empty(int(coff[-1]) + 16, np.uint8)
+    for k in range(ng):
+        a, n = int(gb[k]), int(lens_g[k])
+        cbuf[int(coff[k]):int(coff[k]) + n] = data[a:a + n]
+    wout, wst = orc.unpack_blocks(cbuf[:int(coff[-1])], coff, go, u, is64)
+    gst = st.to_numpy()
+    wcode = np.where((wst & 0xFF) == 99, 1, wst & 0xFF)
+    assert np.array_equal(gst & 0xFF, wcode)
+    assert np.array_equal(gst >> 8, np.where(wcode == 6, wst >> 8, 0))
+    got = out.to_numpy()[:u * item].view(np.uint64 if is64 else np.uint32)
+    dt = np.uint64 if is64 else np.uint32
+    want = np.full(u, np.frombuffer(bytes([POISON] * item), dt)[0], dt)
+    for k in np.nonzero(wst == 0)[0]:
+        a = int(go[k])
+        c = int(cbuf[int(coff[k])]) if lens_g[k] else 0  # count: header byte 0
+        want[a:a + c] = wout[a:a + c]
+    assert np.array_equal(got, want)
+    s.free()
+    cs.free()
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("is64", [False, True])
+def test_fuzz_gathered_damaged(ctx, orc, seed, is64):
+    _case(ctx, orc, 1000 + seed, is64, persistent=bool(seed % 2))
