@@ -13,6 +13,7 @@ its block, so the bytes around it do not matter); a backward range is an
 empty block there, which Unpack rejects with ErrInvalidBuffer as the
 product does. Status words, overflow positions and the values of decoded
 blocks must be identical, and slots of failed blocks keep the poison.
+Batched Gets on the damaged stores are compared the same way.
 """
 import sys
 
@@ -53,11 +54,11 @@ def _values(rng, n, is64):
     return v if is64 else (v & 0xFFFFFFFF).astype(np.uint32)
 
 
-def _case(ctx, orc, seed, is64, persistent):
-    from utlpfor_amd import _lib, batch
-    from utlpfor_amd.device import DeviceArray, check
-    lib = _lib.load()
-    rng = np.random.default_rng(seed)
+def _damaged_store(ctx, rng, is64):
+    """Streams of random flags, lengths and values, encoded on the GPU, with
+    a random share of the blocks damaged byte by byte; returns the device
+    stream over the damaged bytes, those bytes, and the damaged share."""
+    from utlpfor_amd import batch
     lens = np.floor(np.exp(rng.uniform(0, np.log(6000), int(rng.integers(20, 60))))).astype(np.int64)
     flags = rng.choice([0, 1, 2, 3, 4, 5, 6, 7], lens.size).astype(np.uint8)
     v = np.concatenate([_values(rng, int(L), is64) for L in lens])
@@ -71,6 +72,16 @@ def _case(ctx, orc, seed, is64, persistent):
         for _ in range(int(rng.integers(1, 4))):
             data[int(rng.integers(a, b))] = int(rng.integers(0, 256))
     cs = batch.from_host_blocks(ctx, data, meta & 0xFFFFFF, meta >> 24, is64=is64)
+    s.free()
+    return cs, data, lens, share
+
+
+def _case(ctx, orc, seed, is64, persistent):
+    from utlpfor_amd import _lib, batch
+    from utlpfor_amd.device import DeviceArray, check
+    lib = _lib.load()
+    rng = np.random.default_rng(seed)
+    cs, data, lens, share = _damaged_store(ctx, rng, is64)
     ooff = cs.out_off.to_numpy()
     # requests: random ranges, overlapping, repeated, stepping back
     rows, u, e = [], 0, 0
@@ -122,7 +133,6 @@ def _case(ctx, orc, seed, is64, persistent):
         c = int(cbuf[int(coff[k])]) if lens_g[k] else 0  # count: header byte 0
         want[a:a + c] = wout[a:a + c]
     assert np.array_equal(got, want)
-    s.free()
     cs.free()
 
 
@@ -130,3 +140,31 @@ def _case(ctx, orc, seed, is64, persistent):
 @pytest.mark.parametrize("is64", [False, True])
 def test_fuzz_gathered_damaged(ctx, orc, seed, is64):
     _case(ctx, orc, 1000 + seed, is64, persistent=bool(seed % 2))
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("is64", [False, True])
+def test_fuzz_get_damaged(ctx, orc, seed, is64):
+    """Batched Get of (block, position) pairs on a damaged store, positions
+    from -3 to 139 (out of range on both sides), against GetUint32 /
+    GetUint64 of the oracle on each block's bytes."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(2000 + seed)
+    cs, data, lens, share = _damaged_store(ctx, rng, is64)
+    n = 4000
+    blk = rng.integers(0, cs.nblocks, n).astype(np.uint64)
+    pos = rng.integers(-3, 140, n).astype(np.int64)
+    say(f"get fuzz seed {seed} {'u64' if is64 else 'u32'}: {cs.nblocks} blocks ({share:.0%} damaged), {n} Gets")
+    vals, gst = batch.get_batch(ctx, cs, DeviceArray.from_numpy(pos, ctx), blocks=DeviceArray.from_numpy(blk, ctx))
+    ctx.sync()
+    gv, gs = vals.to_numpy(), gst.to_numpy()
+    boff = cs.block_off.to_numpy()
+    get = orc.GetUint64 if is64 else orc.GetUint32
+    for i in range(n):
+        k = int(blk[i])
+        wv, we = get(int(pos[i]), bytes(data[int(boff[k]):int(boff[k + 1])]))
+        assert (gs[i] & 0xFF) == (1 if we == 99 else we), (k, int(pos[i]), we, int(gs[i]))
+        if we == 0:
+            assert int(gv[i]) == wv, (k, int(pos[i]))
+    cs.free()
