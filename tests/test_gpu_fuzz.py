@@ -13,7 +13,8 @@ its block, so the bytes around it do not matter); a backward range is an
 empty block there, which Unpack rejects with ErrInvalidBuffer as the
 product does. Status words, overflow positions and the values of decoded
 blocks must be identical, and slots of failed blocks keep the poison.
-Batched Gets on the damaged stores are compared the same way.
+Batched Gets on the damaged stores are compared the same way, and the GPU
+encoder's bytes with the oracle's on values near its decisions.
 """
 import sys
 
@@ -168,3 +169,54 @@ def test_fuzz_get_damaged(ctx, orc, seed, is64):
         if we == 0:
             assert int(gv[i]) == wv, (k, int(pos[i]))
     cs.free()
+
+
+def _edge_values(rng, n, is64):
+    """Values near the encoder's decisions: exception counts around the
+    list/bitmap switch (16), widths at every multiple of 4, FOR bases near
+    the type's top, u64 blocks that fit 32 bits after FOR, zigzag deltas."""
+    k = int(rng.integers(0, 6))
+    top = 64 if is64 else 32
+    if k == 0:
+        bw = int(rng.integers(0, 9)) * 4
+        v = rng.integers(0, 1 << bw, n, dtype=np.uint64) if bw else np.zeros(n, np.uint64)
+        nexc = int(rng.integers(12, 21))
+        idx = rng.choice(n, min(nexc, n), replace=False)
+        v[idx] = rng.integers(1 << min(bw + 1, top - 2), 1 << (top - 1), idx.size, dtype=np.uint64)
+    elif k == 1:
+        base = (1 << (top - 1)) + int(rng.integers(0, 1 << 20))
+        v = base + rng.integers(0, 1 << int(rng.integers(1, 24)), n, dtype=np.uint64)
+    elif k == 2:
+        v = (np.cumsum(rng.integers(-(1 << 12), 1 << 12, n)) + (1 << 30)).astype(np.int64).astype(np.uint64)
+    elif k == 3:
+        v = rng.integers(0, 1 << 32, n, dtype=np.uint64) + (np.uint64(1 << 40) if is64 else np.uint64(0))
+    elif k == 4:
+        v = np.full(n, rng.integers(0, 1 << (top - 1), dtype=np.uint64), np.uint64)
+    else:
+        v = rng.integers(0, 1 << (top - 1), n, dtype=np.uint64) * 2 + 1
+    return v if is64 else (v & 0xFFFFFFFF).astype(np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("is64", [False, True])
+def test_fuzz_encode_bytes(ctx, orc, seed, is64):
+    """GPU encode of many short streams with every flag combination: the
+    packed bytes equal the oracle's PackUint32/64(flag|Append) stream by
+    stream, and the meta words give its block lengths and counts."""
+    from utlpfor_amd import batch
+    rng = np.random.default_rng(3000 + seed)
+    lens = rng.integers(1, 700, int(rng.integers(30, 80)))
+    flags = rng.choice(8, lens.size).astype(np.uint8)
+    parts = [_edge_values(rng, int(L), is64) for L in lens]
+    say(f"encode fuzz seed {seed} {'u64' if is64 else 'u32'}: {lens.size} streams")
+    s = batch.encode(ctx, np.concatenate(parts), stream_lengths=lens.astype(np.uint64), stream_flags=flags)
+    got, meta = s.bytes_host(), s.meta.to_numpy()
+    off = b = 0
+    for p, f in zip(parts, flags):
+        want, bl, bc = orc.pack_stream(int(f), p)
+        assert np.array_equal(got[off:off + want.size], want), (int(f), p.size)
+        assert np.array_equal(meta[b:b + bl.size] & 0xFFFFFF, bl) and np.array_equal(meta[b:b + bl.size] >> 24, bc)
+        off += want.size
+        b += bl.size
+    assert off == s.nbytes and b == s.nblocks
+    s.free()
