@@ -205,3 +205,49 @@ def test_walk_long_streams_u64(ctx, orc):
     say(f"walk long u64: {len(streams)} streams, {sum(map(len, streams))} bytes")
     counts, status = run_walk(ctx, orc, streams, True)
     assert np.all(status == 0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_walk_fuzz_damaged(ctx, orc, seed):
+    """Random long and short streams (the chunk-parallel and the serial
+    walk), each damaged or not at random: header bytes of random blocks
+    overwritten, random byte flips, truncations, zero runs. The device
+    index equals the host walk stream by stream (starts, lengths, counts,
+    stop status), and so does the decode through it."""
+    rng = np.random.default_rng(900 + seed)
+    streams = []
+    for _ in range(int(rng.integers(3, 7))):
+        n = int(rng.integers(20000, 200000))
+        f = int(rng.choice([0, 1, 2, 4, 3, 5]))
+        if rng.random() < 0.5:
+            v = np.cumsum(rng.integers(0, 1 << int(rng.integers(2, 12)), n)).astype(np.uint32)
+        else:
+            v = rng.integers(0, 1 << int(rng.integers(1, 33)), n, dtype=np.uint64).astype(np.uint32)
+        streams.append(orc.pack_stream(f, v)[0].tobytes())
+    shorts, _ = make_streams(orc, rng, 20, 3000, False)
+    streams += shorts
+    for i in range(len(streams)):
+        a = bytearray(streams[i])
+        if len(a) < 16 or rng.random() < 0.3:
+            continue
+        starts = _starts(orc, bytes(a))
+        for _ in range(int(rng.integers(1, 4))):
+            kind = int(rng.integers(0, 4))
+            if kind == 0 and starts:  # a header byte of a random block
+                j = starts[int(rng.integers(0, len(starts)))] + int(rng.integers(0, 4))
+                if j < len(a):
+                    a[j] = int(rng.integers(0, 256))
+            elif kind == 1:  # a random byte flip
+                j = int(rng.integers(0, len(a)))
+                a[j] ^= 1 << int(rng.integers(0, 8))
+            elif kind == 2:  # truncation
+                a = a[:int(rng.integers(len(a) // 2, len(a)))]
+            elif starts:  # a zero run before a random block
+                j = starts[int(rng.integers(0, len(starts)))]
+                a = a[:j] + bytes(4 * int(rng.integers(1, 3000))) + a[j:]
+        streams[i] = bytes(a)
+    order = rng.permutation(len(streams))
+    streams = [streams[k] for k in order]
+    say(f"walk fuzz seed {seed}: {len(streams)} streams, {sum(map(len, streams))} bytes, "
+        f"{sum(len(b) >= 32768 for b in streams)} long")
+    run_walk(ctx, orc, streams, False)
