@@ -77,3 +77,22 @@ def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(RuntimeError):
         _lib.load()
+
+
+def abi_check_binary():
+    """tests/c_abi/abi_check, built by build() (make) when it is missing."""
+    import subprocess
+    path = os.path.join(ROOT, "tests", "c_abi", "abi_check")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", os.path.dirname(path)], check=True, capture_output=True)
+    return path
+
+
+def test_c_caller_host_entry_points():
+    """The header compiles as C99 with warnings as errors and a C program
+    linked against the library gets the reference's MaxBlockLength known
+    answers and the golden blocks' Header and BlockLength (host reads)."""
+    import subprocess
+    r = subprocess.run([abi_check_binary(), os.path.join(ROOT, "tests", "golden")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
