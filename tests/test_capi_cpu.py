@@ -96,3 +96,53 @@ def test_c_caller_host_entry_points():
     r = subprocess.run([abi_check_binary(), os.path.join(ROOT, "tests", "golden")], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
+
+
+def _c_prototypes():
+    """name -> parameter count of every function declared in include/*.h."""
+    protos = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", fn)).read(), flags=re.S)
+        for m in re.finditer(r"\b(utl_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+            params = m.group(2).strip()
+            protos[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return protos
+
+
+def _call_args(src, start):
+    """Top-level argument count of the call whose '(' is at src[start]."""
+    depth, n, empty = 0, 1, True
+    for ch in src[start:]:
+        if ch in "([{":
+            depth += 1
+            if depth == 1:
+                continue
+        elif ch in ")]}":
+            depth -= 1
+            if depth == 0:
+                return 0 if empty else n
+        elif ch == "," and depth == 1:
+            n += 1
+        if depth >= 1 and not ch.isspace():
+            empty = False
+    raise AssertionError("unbalanced call")
+
+
+def test_go_binding_matches_the_header():
+    """The cgo package written out in INTEGRATION.md (no Go toolchain here to
+    build it) calls only functions the header declares, each with the
+    declared number of arguments, and uses only macros the header defines."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    go = "\n".join(re.findall(r"```go\n(.*?)```", text, flags=re.S))
+    assert "import \"C\"" in go
+    protos = _c_prototypes()
+    calls = [(m.group(1), _call_args(go, m.end() - 1)) for m in re.finditer(r"\bC\.(utl_[a-z0-9_]+)\(", go)]
+    assert len(calls) >= 10
+    for name, nargs in calls:
+        assert name in protos, name
+        assert nargs == protos[name], (name, nargs, protos[name])
+    header = open(os.path.join(ROOT, "include", "utlpfor_gpu.h")).read()
+    for macro in set(re.findall(r"\bC\.(UTL_[A-Z0-9_]+)", go)):
+        assert re.search(rf"#define {macro}\b", header), macro
