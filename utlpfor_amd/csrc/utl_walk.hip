@@ -8,7 +8,7 @@
 // same blocks and statuses.
 //
 // Short streams (< kLong bytes): one wavefront per stream walks it through
-// 16 KiB LDS windows (k_walk_serial).
+// 4 KiB LDS windows (k_walk_serial).
 //
 // Long streams are cut into kChunk-byte chunks and walked chunk-parallel:
 //   k_walk_vote    per chunk, a guess of its entry (the first block start at
@@ -18,17 +18,17 @@
 //                  the chunk, and the crossing most candidates reach wins
 //                  (walks from wrong positions die at a failed header or
 //                  merge into the stream's own chain);
-//   k_walk_chunks  one wavefront per chunk walks it from the guess: block
-//                  count, exit (first position at or past the chunk end) and
-//                  stop status;
-//   k_walk_verify  one wavefront per stream checks, 64 chunks at a time,
+//   k_walk_chunks  one lane per chunk walks it from the guess, reading the
+//                  headers from global memory: block count, exit (first
+//                  position at or past the chunk end) and stop status;
+//   k_walk_verify  one workgroup per stream checks, 1024 chunks at a time,
 //                  that each chunk's guess equals the previous chunk's exit;
 //                  the first chunk that does not is walked again from that
 //                  exit. The result is exact whatever the guesses were: a
 //                  chunk counts only once its entry is the true one, and the
 //                  first chunk's entry is the stream start.
-//   k_walk_emit    (second pass) one wavefront per live chunk walks it again
-//                  and writes its blocks' meta words and starts.
+//   k_walk_emit    (second pass) one lane per live chunk walks it again and
+//                  writes its blocks' meta words and starts.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -60,6 +60,7 @@ struct WalkRec {
   uint32_t pad;
 };
 static_assert(sizeof(WalkRec) == 32, "WalkRec layout");
+static_assert(kLand <= kWarm, "the vote reads kLand jump entries");
 
 // BlockLength (block_length.go:8) of the block whose first bytes are
 // h = bytes 0..3, d1 = bytes 4..7, d2 = bytes 8..11 (little-endian), with
@@ -116,12 +117,10 @@ __device__ __forceinline__ uint64_t stage(uint4* W, const uint8_t* __restrict__ 
     const int c = u * NTHR + tid;
     if (c < CAP16 && c != nfull) W[c] = c < nfull ? v[u] : make_uint4(0u, 0u, 0u, 0u);
   }
-  if (tid == 0 && nfull < CAP16) {  // the unit holding hi: its bytes below hi, zeros after
-    const uint64_t cb = base + 16ull * (uint64_t)nfull;
-    const int nb = hi > cb ? (int)(hi - cb) : 0;  // < 16
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-    for (int k = 0; k < nb; ++k) w[k >> 2] |= (uint32_t)src[cb + k] << (8 * (k & 3));
-    W[nfull] = make_uint4(w[0], w[1], w[2], w[3]);
+  static_assert(NTHR >= 16, "the unit holding hi is read a byte per thread");
+  if (tid < 16 && nfull < CAP16) {  // the unit holding hi: its bytes below hi, zeros after
+    const uint64_t b = base + 16ull * (uint64_t)nfull + (uint64_t)tid;
+    reinterpret_cast<uint8_t*>(W + nfull)[tid] = b < hi ? src[b] : (uint8_t)0;
   }
   return base;
 }
@@ -234,6 +233,71 @@ __device__ __forceinline__ WalkRes walk_chunk(uint4* W, const uint8_t* __restric
                                               uint64_t k, uint64_t pos, uint32_t* __restrict__ meta,
                                               uint64_t* __restrict__ start, uint64_t out) {
   return walk_span(W, src, so, Ls, pos, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, out);
+}
+
+// Bytes 0..11 of a stream from stream byte pos as three dwords, read from
+// global memory by one lane: the four aligned dwords holding them when the
+// stream has at least 16 bytes from pos on, else byte by byte up to the
+// stream's end (zeros after), so no byte past the end is read. src is
+// 16-byte aligned, so the aligned dwords never start before it.
+__device__ __forceinline__ void glob96(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos,
+                                       uint32_t& h, uint32_t& d1, uint32_t& d2) {
+  const uint64_t a = so + pos;
+  if (Ls - pos >= 16) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(src + (a & ~3ull));
+    const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3];
+    const uint32_t sh = (uint32_t)(a & 3);
+    h = __builtin_amdgcn_alignbyte(a1, a0, sh);
+    d1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
+    d2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
+  } else {
+    const int n = (int)(Ls - pos);  // 1..15
+    uint32_t b[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      if (k < n) b[k >> 2] |= (uint32_t)src[a + k] << (8 * (k & 3));
+    h = b[0];
+    d1 = b[1];
+    d2 = b[2];
+  }
+}
+
+// walk_span for one lane: walks the stream (absolute start so, length Ls)
+// from pos while pos < end, reading each header from global memory. With
+// meta set it writes block i's meta word to meta[out + i] and its absolute
+// start to start[out + i]. Every step advances by at least 4 bytes.
+__device__ WalkRes walk_lane(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos, uint64_t end,
+                             uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
+  uint32_t n = 0, st = kStLand;
+  while (pos < end) {
+    uint32_t h, d1, d2;
+    glob96(src, so, Ls, pos, h, d1, d2);
+    const uint64_t left = Ls - pos;
+    bool ok;
+    uint32_t bl = fast_len(h, d1, d2, &ok);
+    if (!ok || left < 16u) {  // the reference's checks in order
+      const int err = step_len(h, d1, d2, (int64_t)left, &bl);
+      if (err != UTL_OK) {
+        st = kStStop | (uint32_t)err;
+        break;
+      }
+    }
+    if (bl > left) {
+      st = kStStop | UTL_ERR_INVALID_BUFFER;
+      break;
+    }
+    if (meta) {
+      meta[out + n] = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
+      if (start) start[out + n] = so + pos;
+    }
+    ++n;
+    pos += bl;
+    if (pos == Ls) {
+      st = kStEnd;
+      break;
+    }
+  }
+  return {pos, n, st};
 }
 
 __device__ __forceinline__ uint64_t stream_len(const uint64_t* stream_off, uint64_t s, uint64_t* so) {
@@ -380,14 +444,16 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
   }
 }
 
-// Walk of every chunk from its voted entry (one wavefront per chunk).
+// Walk of every chunk from its voted entry.
 __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
                                                     WalkRec* __restrict__ recs) {
-  __shared__ uint4 W[kWin16];
+  // one lane per chunk: the chunks' chains of dependent header loads run
+  // side by side instead of one wavefront stepping through one chunk
   const uint64_t total = min(chunk_base[ns], max_chunks);
-  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += step) {
     const uint64_t s = owner(chunk_base, ns, c);
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
@@ -395,12 +461,10 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
     const uint64_t k = c - chunk_base[s];
     const uint64_t e = recs[c].e;
     WalkRes r{kNone, 0u, kStLand};
-    if (e != kNone) r = walk_chunk(W, src, so, Ls, k, e, nullptr, nullptr, 0);
-    if (lane_id() == 0) {
-      recs[c].x = r.x;
-      recs[c].n = r.n;
-      recs[c].st = r.st;
-    }
+    if (e != kNone) r = walk_lane(src, so, Ls, e, min((k + 1) * (uint64_t)kChunk, Ls), nullptr, nullptr, 0);
+    recs[c].x = r.x;
+    recs[c].n = r.n;
+    recs[c].st = r.st;
   }
 }
 
@@ -511,16 +575,16 @@ __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict_
   }
 }
 
-// Meta words and block starts of every live chunk (one wavefront per chunk).
+// Meta words and block starts of every live chunk (one lane per chunk).
 __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ src,
                                                   const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                   const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
                                                   const WalkRec* __restrict__ recs, const uint32_t* __restrict__ term,
                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
                                                   uint64_t* __restrict__ start) {
-  __shared__ uint4 W[kWin16];
   const uint64_t total = min(chunk_base[ns], max_chunks);
-  for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += step) {
     const uint64_t s = owner(chunk_base, ns, c);
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
@@ -529,7 +593,7 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
     if (k > term[s]) continue;
     const WalkRec r = recs[c];
     if (r.n == 0) continue;
-    walk_chunk(W, src, so, Ls, k, r.e, meta, start, blk_off[s] + r.base);
+    walk_lane(src, so, Ls, r.e, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, blk_off[s] + r.base);
   }
 }
 
@@ -583,11 +647,12 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
     const unsigned cgrid = (unsigned)min(max_chunks, (uint64_t)16384);
     k_walk_vote<<<dim3(min(cgrid, 4096u)), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base,
                                                                     max_chunks, recs);
-    k_walk_chunks<<<dim3(cgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs);
+    const unsigned lgrid = (unsigned)min((max_chunks + 63) / 64, (uint64_t)4096);  // one lane per chunk
+    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs);
     k_walk_verify<<<dim3(min(sgrid, 4096u)), dim3(kVer), 0, stream>>>(src, stream_off, nstreams, chunk_base,
                                                                        max_chunks, recs, term, nblocks, status);
     if (meta)
-      k_walk_emit<<<dim3(cgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs,
+      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs,
                                                          term, blk_off, meta, start);
   }
   k_walk_serial<<<dim3(sgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
