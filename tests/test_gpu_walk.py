@@ -196,6 +196,22 @@ def test_walk_long_streams_chunked(ctx, orc):
     assert counts[streams.index(zero_run)] == counts[streams.index(longs[0])] + 5000
 
 
+def test_walk_many_streams_long_in_every_tile(ctx, orc):
+    """More streams than one tile of the chunk-offset scan (1024 streams):
+    long streams in the first, middle and last tiles, the last stream long,
+    between short and empty ones, so that every tile offset is used."""
+    rng = np.random.default_rng(26)
+    streams, _ = make_streams(orc, rng, 3000, 200, False)
+    for i in (3, 1023, 1024, 1500, 2047, 2999):
+        n = int(rng.integers(40000, 120000))
+        streams[i] = orc.pack_stream(int(rng.choice([0, 1, 2])),
+                                     np.cumsum(rng.integers(0, 1 << 8, n)).astype(np.uint32))[0].tobytes()
+    say(f"walk many: {len(streams)} streams, {sum(map(len, streams))} bytes")
+    counts, status = run_walk(ctx, orc, streams, False)
+    assert np.all(status == 0)
+    assert sum(len(b) >= 32768 for b in streams) == 6
+
+
 def test_walk_long_streams_u64(ctx, orc):
     rng = np.random.default_rng(25)
     ts = (int(1.7e18) + np.cumsum(rng.geometric(1 / 1000, 200000))).astype(np.uint64)

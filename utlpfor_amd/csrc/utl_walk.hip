@@ -7,8 +7,8 @@
 // longer than the bytes left (ErrInvalidBuffer). The kernels here give the
 // same blocks and statuses.
 //
-// Short streams (< kLong bytes): one wavefront per stream walks it through
-// 4 KiB LDS windows (k_walk_serial).
+// Short streams (< kLong bytes): one lane per stream walks it, reading the
+// headers from global memory (k_walk_serial).
 //
 // Long streams are cut into kChunk-byte chunks and walked chunk-parallel:
 //   k_walk_vote    per chunk, a guess of its entry (the first block start at
@@ -235,69 +235,114 @@ __device__ __forceinline__ WalkRes walk_chunk(uint4* W, const uint8_t* __restric
   return walk_span(W, src, so, Ls, pos, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, out);
 }
 
-// Bytes 0..11 of a stream from stream byte pos as three dwords, read from
-// global memory by one lane: the four aligned dwords holding them when the
-// stream has at least 16 bytes from pos on, else byte by byte up to the
-// stream's end (zeros after), so no byte past the end is read. src is
-// 16-byte aligned, so the aligned dwords never start before it.
-__device__ __forceinline__ void glob96(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos,
-                                       uint32_t& h, uint32_t& d1, uint32_t& d2) {
+// The dwords holding bytes 0..11 of a stream from stream byte pos, read
+// from global memory by one lane: the four aligned dwords around them when
+// the stream has at least 16 bytes from pos on (sh: the byte offset in the
+// first), else the bytes up to the stream's end with zeros after (sh 0), so
+// no byte past the end is read. src is 16-byte aligned, so the aligned
+// dwords never start before it. The words are only combined by hdr96, which
+// lets a caller issue the load and use its registers later.
+struct Raw96 {
+  uint32_t a0, a1, a2, a3, sh;
+};
+
+__device__ __forceinline__ Raw96 load96(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos) {
   const uint64_t a = so + pos;
+  Raw96 r;
   if (Ls - pos >= 16) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(src + (a & ~3ull));
-    const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3];
-    const uint32_t sh = (uint32_t)(a & 3);
-    h = __builtin_amdgcn_alignbyte(a1, a0, sh);
-    d1 = __builtin_amdgcn_alignbyte(a2, a1, sh);
-    d2 = __builtin_amdgcn_alignbyte(a3, a2, sh);
+    r.a0 = w[0];
+    r.a1 = w[1];
+    r.a2 = w[2];
+    r.a3 = w[3];
+    r.sh = (uint32_t)(a & 3);
   } else {
     const int n = (int)(Ls - pos);  // 1..15
     uint32_t b[3] = {0u, 0u, 0u};
 #pragma unroll
     for (int k = 0; k < 12; ++k)
       if (k < n) b[k >> 2] |= (uint32_t)src[a + k] << (8 * (k & 3));
-    h = b[0];
-    d1 = b[1];
-    d2 = b[2];
+    r.a0 = b[0];
+    r.a1 = b[1];
+    r.a2 = b[2];
+    r.a3 = 0u;
+    r.sh = 0u;
   }
+  return r;
+}
+
+__device__ __forceinline__ void hdr96(const Raw96& r, uint32_t& h, uint32_t& d1, uint32_t& d2) {
+  h = __builtin_amdgcn_alignbyte(r.a1, r.a0, r.sh);
+  d1 = __builtin_amdgcn_alignbyte(r.a2, r.a1, r.sh);
+  d2 = __builtin_amdgcn_alignbyte(r.a3, r.a2, r.sh);
+}
+
+// What a lane walk writes per block: nothing (counts only), the meta word,
+// or the meta word and the block's absolute start.
+enum LaneOut { kCount = 0, kMeta = 1, kMetaStart = 2 };
+
+// One block of a lane walk: the header in r (at pos) is checked, the next
+// header's load is issued, then the block's outputs are stored. Returns
+// whether the walk goes on (st is set when it stops).
+template <int OUT>
+__device__ __forceinline__ bool lane_step(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t end,
+                                          uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out,
+                                          Raw96& r, uint64_t& pos, uint32_t& n, uint32_t& st) {
+  uint32_t h, d1, d2;
+  hdr96(r, h, d1, d2);
+  const uint64_t left = Ls - pos;
+  bool ok;
+  uint32_t bl = fast_len(h, d1, d2, &ok);
+  if (!ok || left < 16u) {  // the reference's checks in order
+    const int err = step_len(h, d1, d2, (int64_t)left, &bl);
+    if (err != UTL_OK) {
+      st = kStStop | (uint32_t)err;
+      return false;
+    }
+  }
+  if (bl > left) {
+    st = kStStop | UTL_ERR_INVALID_BUFFER;
+    return false;
+  }
+  const uint64_t next = pos + bl;
+  const bool more = next < end && next != Ls;
+  const uint32_t m = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
+  if (more) r = load96(src, so, Ls, next);
+  if (OUT >= kMeta) meta[out + n] = m;
+  if (OUT >= kMetaStart) start[out + n] = so + pos;
+  ++n;
+  pos = next;
+  if (!more && pos == Ls) st = kStEnd;
+  return more;
 }
 
 // walk_span for one lane: walks the stream (absolute start so, length Ls)
-// from pos while pos < end, reading each header from global memory. With
-// meta set it writes block i's meta word to meta[out + i] and its absolute
-// start to start[out + i]. Every step advances by at least 4 bytes.
+// from pos while pos < end, reading each header from global memory. OUT
+// kMeta writes block i's meta word to meta[out + i], kMetaStart also its
+// absolute start to start[out + i]. Every step advances by at least 4
+// bytes. Each step issues the next header's load before its own stores and
+// combines the words after them: one counter tracks loads and stores in
+// issue order, so the wait for the load leaves the stores in flight. The
+// first step is peeled so that every entry to the loop has that same order.
+template <int OUT>
 __device__ WalkRes walk_lane(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos, uint64_t end,
                              uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
   uint32_t n = 0, st = kStLand;
-  while (pos < end) {
-    uint32_t h, d1, d2;
-    glob96(src, so, Ls, pos, h, d1, d2);
-    const uint64_t left = Ls - pos;
-    bool ok;
-    uint32_t bl = fast_len(h, d1, d2, &ok);
-    if (!ok || left < 16u) {  // the reference's checks in order
-      const int err = step_len(h, d1, d2, (int64_t)left, &bl);
-      if (err != UTL_OK) {
-        st = kStStop | (uint32_t)err;
-        break;
-      }
+  if (pos >= end) return {pos, n, st};
+  Raw96 r = load96(src, so, Ls, pos);
+  if (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st))
+    while (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st)) {
     }
-    if (bl > left) {
-      st = kStStop | UTL_ERR_INVALID_BUFFER;
-      break;
-    }
-    if (meta) {
-      meta[out + n] = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
-      if (start) start[out + n] = so + pos;
-    }
-    ++n;
-    pos += bl;
-    if (pos == Ls) {
-      st = kStEnd;
-      break;
-    }
-  }
   return {pos, n, st};
+}
+
+// walk_lane with the output form chosen by which pointers are set.
+__device__ __forceinline__ WalkRes walk_lane_out(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls,
+                                                 uint64_t pos, uint64_t end, uint32_t* __restrict__ meta,
+                                                 uint64_t* __restrict__ start, uint64_t out) {
+  if (!meta) return walk_lane<kCount>(src, so, Ls, pos, end, nullptr, nullptr, 0);
+  if (!start) return walk_lane<kMeta>(src, so, Ls, pos, end, meta, nullptr, out);
+  return walk_lane<kMetaStart>(src, so, Ls, pos, end, meta, start, out);
 }
 
 __device__ __forceinline__ uint64_t stream_len(const uint64_t* stream_off, uint64_t s, uint64_t* so) {
@@ -327,16 +372,35 @@ __device__ __forceinline__ int32_t status_of(uint32_t st) {
   return st >= kStStop ? (int32_t)(st & 0xFFu) : UTL_OK;
 }
 
-// Chunks per stream (0 for short streams), scanned in place by k_walk_scan.
-__global__ void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns, uint64_t* __restrict__ q) {
-  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > ns) return;
+// Chunk offsets of the streams: chunks per stream (0 for short streams),
+// scanned within tiles of kPlan streams by k_walk_plan; with more than one
+// tile, k_walk_scan scans the tile totals and k_walk_apply adds them.
+constexpr int kPlan = 1024;
+
+// chunk_base[s] = chunks of the streams of s's tile before s (s <= ns; the
+// entry ns closes the last stream); tile_sum[t] = tile t's total.
+__global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                     uint64_t* __restrict__ chunk_base, uint64_t* __restrict__ tile_sum) {
+  __shared__ uint64_t red[kPlan / 64];
+  const int tid = threadIdx.x, wid = tid >> 6;
+  const uint64_t s = (uint64_t)blockIdx.x * kPlan + (uint64_t)tid;
   uint64_t so;
   const uint64_t Ls = s < ns ? stream_len(stream_off, s, &so) : 0;
-  q[s] = Ls >= kLong ? (Ls + kChunk - 1) / kChunk : 0;
+  const uint64_t c = Ls >= kLong ? (Ls + kChunk - 1) / kChunk : 0;
+  const uint64_t incl = wave_incl_scan64(c);
+  if (lane_id() == 63) red[wid] = incl;
+  __syncthreads();
+  uint64_t before = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < kPlan / 64; ++w) {
+    before += w < wid ? red[w] : 0u;
+    sum += red[w];
+  }
+  if (s <= ns) chunk_base[s] = before + incl - c;
+  if (tid == 0) tile_sum[blockIdx.x] = sum;
 }
 
-// Exclusive scan of q[0..n] in place (one workgroup of 1024 threads).
+// Exclusive scan of q[0..n) in place (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
   __shared__ uint64_t part[1024];
   const uint64_t per = (n + 1023) / 1024;
@@ -357,6 +421,14 @@ __global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
     q[i] = run;
     run += v;
   }
+}
+
+// Adds the offsets of tiles 1.. to their streams' entries.
+__global__ __launch_bounds__(kPlan) void k_walk_apply(uint64_t* __restrict__ chunk_base, uint64_t ns,
+                                                      const uint64_t* __restrict__ tile_off) {
+  const uint64_t t = (uint64_t)blockIdx.x + 1;
+  const uint64_t s = t * kPlan + threadIdx.x;
+  if (s <= ns) chunk_base[s] += tile_off[t];
 }
 
 // Entry vote of chunk k >= 1 (one workgroup of 256 threads per chunk).
@@ -461,7 +533,7 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
     const uint64_t k = c - chunk_base[s];
     const uint64_t e = recs[c].e;
     WalkRes r{kNone, 0u, kStLand};
-    if (e != kNone) r = walk_lane(src, so, Ls, e, min((k + 1) * (uint64_t)kChunk, Ls), nullptr, nullptr, 0);
+    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * (uint64_t)kChunk, Ls), nullptr, nullptr, 0);
     recs[c].x = r.x;
     recs[c].n = r.n;
     recs[c].st = r.st;
@@ -593,27 +665,25 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
     if (k > term[s]) continue;
     const WalkRec r = recs[c];
     if (r.n == 0) continue;
-    walk_lane(src, so, Ls, r.e, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, blk_off[s] + r.base);
+    walk_lane_out(src, so, Ls, r.e, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, blk_off[s] + r.base);
   }
 }
 
-// Serial walk of the short streams (one wavefront per stream).
-__global__ __launch_bounds__(64) void k_walk_serial(const uint8_t* __restrict__ src,
-                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
-                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                    const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
-                                                    uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
-                                                    int32_t* __restrict__ status) {
-  __shared__ uint4 W[kWin16];
-  for (uint64_t s = blockIdx.x; s < ns; s += gridDim.x) {
+// Serial walk of the short streams (one lane per stream).
+__global__ __launch_bounds__(256) void k_walk_serial(const uint8_t* __restrict__ src,
+                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                     const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
+                                                     uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
+                                                     int32_t* __restrict__ status) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += step) {
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (chunk_base && is_long(Ls, chunk_base, s, max_chunks)) continue;
-    const WalkRes r = walk_span(W, src, so, Ls, 0, Ls, meta, start, meta ? blk_off[s] : 0);
-    if (lane_id() == 0) {
-      nblocks[s] = r.n;
-      status[s] = status_of(r.st);
-    }
+    const WalkRes r = walk_lane_out(src, so, Ls, 0, Ls, meta, start, meta ? blk_off[s] : 0);
+    nblocks[s] = r.n;
+    status[s] = status_of(r.st);
   }
 }
 
@@ -622,9 +692,13 @@ __global__ __launch_bounds__(64) void k_walk_serial(const uint8_t* __restrict__ 
 
 using namespace utl;
 
+// Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
+// records, term (one word per stream).
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
-  return ((nstreams + 1) * 8 + 255) / 256 * 256 + max_chunks * sizeof(WalkRec) + nstreams * 4 + 256;
+  const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
+  return ((nstreams + 1) * 8 + 255) / 256 * 256 + (ntiles * 8 + 255) / 256 * 256 + max_chunks * sizeof(WalkRec) +
+         nstreams * 4 + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
@@ -632,18 +706,23 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
                                  uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream) {
   if (nstreams == 0) return hipSuccess;
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
+  const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   uint8_t* t = static_cast<uint8_t*>(tmp);
   uint64_t* chunk_base = reinterpret_cast<uint64_t*>(t);
   t += ((nstreams + 1) * 8 + 255) / 256 * 256;
+  uint64_t* tile_sum = reinterpret_cast<uint64_t*>(t);
+  t += (ntiles * 8 + 255) / 256 * 256;
   WalkRec* recs = reinterpret_cast<WalkRec*>(t);
   t += max_chunks * sizeof(WalkRec);
   uint32_t* term = reinterpret_cast<uint32_t*>(t);
   const bool chunked = total_bytes >= kLong;
   const unsigned sgrid = (unsigned)min(nstreams, (uint64_t)16384);
   if (chunked) {
-    k_walk_plan<<<dim3((unsigned)((nstreams + 1 + 255) / 256)), dim3(256), 0, stream>>>(stream_off, nstreams,
-                                                                                         chunk_base);
-    k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(chunk_base, nstreams + 1);
+    k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(stream_off, nstreams, chunk_base, tile_sum);
+    if (ntiles > 1) {
+      k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(tile_sum, ntiles);
+      k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
+    }
     const unsigned cgrid = (unsigned)min(max_chunks, (uint64_t)16384);
     k_walk_vote<<<dim3(min(cgrid, 4096u)), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base,
                                                                     max_chunks, recs);
@@ -655,7 +734,8 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
       k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs,
                                                          term, blk_off, meta, start);
   }
-  k_walk_serial<<<dim3(sgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
-                                                       max_chunks, blk_off, meta, start, nblocks, status);
+  const unsigned lsgrid = (unsigned)min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
+  k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
+                                                         max_chunks, blk_off, meta, start, nblocks, status);
   return hipGetLastError();
 }
