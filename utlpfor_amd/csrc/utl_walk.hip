@@ -431,17 +431,38 @@ __global__ __launch_bounds__(kPlan) void k_walk_apply(uint64_t* __restrict__ chu
   if (s <= ns) chunk_base[s] += tile_off[t];
 }
 
-// Entry vote of chunk k >= 1 (one workgroup of 256 threads per chunk).
+// Whether header h passes the checks of BlockLength that do not depend on
+// the bytes left (count, bit width, flags). A header failing them ends
+// every walk that reaches it.
+__device__ __forceinline__ bool hdr_ok(uint32_t h) {
+  const bool t3 = ((h >> 13) & 3u) == 3u;
+  const uint32_t bad = h & (kHReserved | kHBlock256 | (t3 ? 0u : kHCombine));
+  return (h & 0xFFu) <= (uint32_t)kBlock && ((h >> 8) & 31u) <= 8u && bad == 0u;
+}
+
+constexpr int kVotePer = kWarm / 256;  // warm-up positions per thread of a vote
+static_assert(kVotePer == 8 && kLand == kWarm, "a vote thread owns 8 consecutive positions, one 16-byte J row");
+
+// Entry vote of chunk k >= 1 (one workgroup of 256 threads per chunk;
+// thread t owns the warm-up positions 8t .. 8t+7). The header checks run on
+// all positions from three shared dwords; BlockLength is computed only for
+// the positions that pass them, and pointer jumping and voting loop over a
+// thread's live positions only (a few per cent of all positions on packed
+// data: every other position is dead after one hop).
 __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ src,
                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
                                                    WalkRec* __restrict__ recs) {
   __shared__ uint4 W[kWarm16];
-  __shared__ uint16_t J[kWarm];
+  __shared__ uint4 J4[kWarm / 8];  // J[p]: next hop of position p (16-bit entries)
   __shared__ uint32_t hist[kLand];
   __shared__ uint32_t best[4];
+  uint16_t* J = reinterpret_cast<uint16_t*>(J4);
   const uint64_t total = min(chunk_base[ns], max_chunks);
   const int tid = threadIdx.x;
+  const int p0 = kVotePer * tid;
+  // cleared once; each chunk clears the entries it counted in
+  for (int i = tid; i < kLand; i += 256) hist[i] = 0;
   for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
     const uint64_t s = owner(chunk_base, ns, c);
     uint64_t so;
@@ -455,55 +476,74 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
     const uint64_t cs = k * (uint64_t)kChunk;  // chunk start (stream-relative)
     const uint64_t ws = cs - kWarm;            // warm-up start
     const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
-    for (int i = tid; i < kLand; i += 256) hist[i] = 0;
     __syncthreads();
     const uint32_t* W32 = reinterpret_cast<const uint32_t*>(W);
-    // first hop of every warm-up position: another warm-up position, a
-    // crossing into the chunk (kWarm + offset), or dead
-    for (int p = tid; p < kWarm; p += 256) {
+    // window byte of position p0 (at most 15 + 2040: the dwords read below
+    // end before byte 2072 of the 2112-byte window)
+    const int b0 = (int)(so + ws - wabs) + p0;
+    uint32_t cand = 0;
+    {
+      const int w0 = b0 >> 2;
+      const uint32_t sh = (uint32_t)(b0 & 3);
+      const uint32_t x0 = W32[w0], x1 = W32[w0 + 1], x2 = W32[w0 + 2], x3 = W32[w0 + 3];
+      const uint32_t A0 = __builtin_amdgcn_alignbyte(x1, x0, sh);  // bytes b0 .. b0+3
+      const uint32_t A1 = __builtin_amdgcn_alignbyte(x2, x1, sh);
+      const uint32_t A2 = __builtin_amdgcn_alignbyte(x3, x2, sh);
+#pragma unroll
+      for (int u = 0; u < kVotePer; ++u) {
+        const uint32_t h = __builtin_amdgcn_alignbyte(u < 4 ? A1 : A2, u < 4 ? A0 : A1, (uint32_t)(u & 3));
+        cand |= hdr_ok(h) ? 1u << u : 0u;
+      }
+    }
+    // first hop: another warm-up position, a crossing into the chunk
+    // (kWarm + offset), or dead
+    J4[tid] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    uint32_t live = 0, open = 0;
+    for (uint32_t m = cand; m; m &= m - 1) {
+      const int u = __builtin_ctz(m);
+      const int p = p0 + u;
       uint32_t h, d1, d2;
-      win96(W32, (int)(so + ws + p - wabs), h, d1, d2);
-      const int64_t left = (int64_t)(Ls - (ws + p));
+      win96(W32, b0 + u, h, d1, d2);
+      // bytes to the stream end (at least 1: ws + p < cs < Ls), capped: the
+      // checks compare it with lengths below 2^18
+      const int64_t left = (int64_t)min(Ls - (ws + (uint64_t)p), (uint64_t)0x7FFFFFFF);
       uint32_t bl;
       const int err = step_len(h, d1, d2, left, &bl);
-      uint16_t j = kDead;
-      if (err == UTL_OK && (int64_t)bl < left) {  // bl == left ends the stream: no entry in this chunk
-        const uint64_t q = (uint64_t)p + bl;
-        if (q < (uint64_t)kWarm) j = (uint16_t)q;
-        else if (q - kWarm < (uint64_t)kLand) j = (uint16_t)q;
-      }
-      J[p] = j;
-    }
-    __syncthreads();
-    // pointer jumping: each hop doubles; (index, value) updates are single
-    // 16-bit words, so a stale read is still a position on the same path
-    constexpr int kPer = kWarm / 256;
-    for (;;) {
-      uint32_t v[kPer], t[kPer];
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) v[u] = J[u * 256 + tid];
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) t[u] = v[u] < (uint32_t)kWarm ? J[v[u]] : v[u];
-      int changed = 0;
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        if (t[u] != v[u]) {
-          J[u * 256 + tid] = (uint16_t)t[u];
-          changed = 1;
+      // bl == left ends the stream: no entry in this chunk
+      if (err == UTL_OK && (int64_t)bl < left) {
+        const uint32_t q = (uint32_t)p + bl;
+        if (q < (uint32_t)(kWarm + kLand)) {
+          J[p] = (uint16_t)q;
+          live |= 1u << u;
+          if (q < (uint32_t)kWarm) open |= 1u << u;
         }
       }
-      if (!__syncthreads_or(changed)) break;
     }
-    for (int o = tid; o < kLand; o += 256) {
-      const uint16_t v = J[o];
+    __syncthreads();
+    // pointer jumping over the open positions (next hop still inside the
+    // warm-up): each hop doubles; (index, value) updates are single 16-bit
+    // words, so a stale read is still a position on the same path. Only
+    // position p's thread writes J[p].
+    for (;;) {
+      for (uint32_t m = open; m; m &= m - 1) {
+        const int u = __builtin_ctz(m);
+        const uint32_t v = J[p0 + u];  // < kWarm
+        const uint32_t t = J[v];
+        J[p0 + u] = (uint16_t)t;
+        if (t >= (uint32_t)kWarm) open &= ~(1u << u);  // crossed, or dead (kDead)
+      }
+      if (!__syncthreads_or(open != 0u)) break;
+    }
+    for (uint32_t m = live; m; m &= m - 1) {
+      const uint32_t v = J[p0 + __builtin_ctz(m)];
       if (v != kDead) atomicAdd(&hist[v - kWarm], 1u);
     }
     __syncthreads();
     // most votes; ties to the smallest offset
     uint32_t b = 0;
-    for (int i = tid; i < kLand; i += 256) {
-      const uint32_t key = hist[i] ? ((hist[i] << 16) | (uint32_t)(0xFFFF - i)) : 0u;
-      b = max(b, key);
+    for (uint32_t m = live; m; m &= m - 1) {
+      const uint32_t v = J[p0 + __builtin_ctz(m)];
+      if (v != kDead) b = max(b, (hist[v - kWarm] << 16) | (0xFFFFu - (v - kWarm)));
     }
     b = wave_reduce_max(b);
     if (lane_id() == 0) best[tid >> 6] = b;
@@ -511,6 +551,10 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
     if (tid == 0) {
       const uint32_t m = max(max(best[0], best[1]), max(best[2], best[3]));
       recs[c].e = m ? cs + (uint64_t)(0xFFFF - (m & 0xFFFF)) : kNone;
+    }
+    for (uint32_t m = live; m; m &= m - 1) {
+      const uint32_t v = J[p0 + __builtin_ctz(m)];
+      if (v != kDead) hist[v - kWarm] = 0;
     }
     __syncthreads();
   }
