@@ -212,6 +212,47 @@ def test_walk_many_streams_long_in_every_tile(ctx, orc):
     assert sum(len(b) >= 32768 for b in streams) == 6
 
 
+def test_walk_large_input_longer_chunks(ctx, orc):
+    """A walk over more than 113 MB cuts long streams into chunks longer than
+    16,320 bytes (the chunk length grows with the square root of the bytes
+    walked). 4000 uneven streams encoded on the device (mixed flags, lengths
+    up to 200,000 values), with the count byte of a middle block of five
+    long streams set to 200: the other streams' walked blocks equal the
+    encoder's index, the damaged ones stop where the oracle's walk stops."""
+    from utlpfor_amd import batch, workloads
+    from utlpfor_amd.device import DeviceArray
+    v, lens, flags = workloads.uneven_streams(4000, seed=7)
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    assert s.nbytes > 120_000_000
+    data = s.bytes_host().copy()
+    boff, meta = s.block_off.to_numpy(), s.meta.to_numpy()
+    sb = batch.plan_blocks(lens).astype(np.int64)
+    stream_off = boff[sb]
+    big = np.nonzero(np.diff(stream_off) > 200_000)[0]  # more than ten chunks each
+    damaged = [int(i) for i in big[::max(1, big.size // 5)][:5]]
+    assert len(damaged) == 5
+    for i in damaged:
+        data[int(boff[sb[i] + (sb[i + 1] - sb[i]) // 2])] = 200
+    say(f"walk large: {lens.size} streams, {s.nbytes} bytes, {s.nblocks} blocks, damaged {damaged}")
+    d = DeviceArray.from_numpy(np.concatenate([data, np.zeros(16, np.uint8)]), ctx)
+    w, counts, status = batch.index_foreign(ctx, d, stream_off)
+    want_n = np.diff(sb)
+    want_st = np.zeros(lens.size, np.int32)
+    for i in damaged:
+        st0, _, _, st = host_walk(orc, bytes(data[int(stream_off[i]):int(stream_off[i + 1])]))
+        want_n[i], want_st[i] = len(st0), st
+    assert np.array_equal(counts, want_n) and np.array_equal(status, want_st)
+    assert all(want_st[i] == 3 for i in damaged)  # ErrInvalidBlockLength
+    keep = np.concatenate([np.arange(sb[i], sb[i] + want_n[i]) for i in range(lens.size)])
+    nb = keep.size
+    assert w.nblocks == nb
+    assert np.array_equal(w.block_off.to_numpy(nb), boff[keep])
+    assert np.array_equal(w.meta.to_numpy(nb), meta[keep])
+    w.free()
+    d.free()
+    s.free()
+
+
 def test_walk_long_streams_u64(ctx, orc):
     rng = np.random.default_rng(25)
     ts = (int(1.7e18) + np.cumsum(rng.geometric(1 / 1000, 200000))).astype(np.uint64)

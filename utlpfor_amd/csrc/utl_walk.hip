@@ -10,7 +10,8 @@
 // Short streams (< kLong bytes): one lane per stream walks it, reading the
 // headers from global memory (k_walk_serial).
 //
-// Long streams are cut into kChunk-byte chunks and walked chunk-parallel:
+// Long streams are cut into chunks (walk_chunk_len bytes, at least kChunk)
+// and walked chunk-parallel:
 //   k_walk_vote    per chunk, a guess of its entry (the first block start at
 //                  or after the chunk start): every candidate position of a
 //                  kWarm-byte warm-up region before the chunk is followed by
@@ -32,6 +33,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "utl_device.h"
 #include "utl_format.h"
 #include "utl_kernels.h"
@@ -39,7 +43,7 @@
 namespace utl {
 namespace {
 
-constexpr int kChunk = 16320;            // bytes of a long stream per chunk (window: 16 KiB)
+constexpr int kChunk = 16320;            // least bytes of a long stream per chunk
 constexpr int kWarm = 2048;              // warm-up bytes before a chunk
 constexpr int kLand = 2048;              // entry candidates (the warm-up's first bytes); above every block
                                          // length an encoder emits (2038)
@@ -230,9 +234,9 @@ __device__ WalkRes walk_span(uint4* W, const uint8_t* __restrict__ src, uint64_t
 
 // Walks chunk k of a long stream from pos (a wave-collective call).
 __device__ __forceinline__ WalkRes walk_chunk(uint4* W, const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls,
-                                              uint64_t k, uint64_t pos, uint32_t* __restrict__ meta,
+                                              uint64_t chunk, uint64_t k, uint64_t pos, uint32_t* __restrict__ meta,
                                               uint64_t* __restrict__ start, uint64_t out) {
-  return walk_span(W, src, so, Ls, pos, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, out);
+  return walk_span(W, src, so, Ls, pos, min((k + 1) * chunk, Ls), meta, start, out);
 }
 
 // The dwords holding bytes 0..11 of a stream from stream byte pos, read
@@ -380,13 +384,14 @@ constexpr int kPlan = 1024;
 // chunk_base[s] = chunks of the streams of s's tile before s (s <= ns; the
 // entry ns closes the last stream); tile_sum[t] = tile t's total.
 __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns,
-                                                     uint64_t* __restrict__ chunk_base, uint64_t* __restrict__ tile_sum) {
+                                                     uint64_t chunk, uint64_t* __restrict__ chunk_base,
+                                                     uint64_t* __restrict__ tile_sum) {
   __shared__ uint64_t red[kPlan / 64];
   const int tid = threadIdx.x, wid = tid >> 6;
   const uint64_t s = (uint64_t)blockIdx.x * kPlan + (uint64_t)tid;
   uint64_t so;
   const uint64_t Ls = s < ns ? stream_len(stream_off, s, &so) : 0;
-  const uint64_t c = Ls >= kLong ? (Ls + kChunk - 1) / kChunk : 0;
+  const uint64_t c = Ls >= kLong ? (Ls + chunk - 1) / chunk : 0;
   const uint64_t incl = wave_incl_scan64(c);
   if (lane_id() == 63) red[wid] = incl;
   __syncthreads();
@@ -452,7 +457,7 @@ static_assert(kVotePer == 8 && kLand == kWarm, "a vote thread owns 8 consecutive
 __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ src,
                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                   WalkRec* __restrict__ recs) {
+                                                   uint64_t chunk, WalkRec* __restrict__ recs) {
   __shared__ uint4 W[kWarm16];
   __shared__ uint4 J4[kWarm / 8];  // J[p]: next hop of position p (16-bit entries)
   __shared__ uint32_t hist[kLand];
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
       if (tid == 0) recs[c].e = 0;
       continue;
     }
-    const uint64_t cs = k * (uint64_t)kChunk;  // chunk start (stream-relative)
+    const uint64_t cs = k * chunk;  // chunk start (stream-relative)
     const uint64_t ws = cs - kWarm;            // warm-up start
     const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
     __syncthreads();
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
 __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                    WalkRec* __restrict__ recs) {
+                                                    uint64_t chunk, WalkRec* __restrict__ recs) {
   // one lane per chunk: the chunks' chains of dependent header loads run
   // side by side instead of one wavefront stepping through one chunk
   const uint64_t total = min(chunk_base[ns], max_chunks);
@@ -577,7 +582,7 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
     const uint64_t k = c - chunk_base[s];
     const uint64_t e = recs[c].e;
     WalkRes r{kNone, 0u, kStLand};
-    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * (uint64_t)kChunk, Ls), nullptr, nullptr, 0);
+    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * chunk, Ls), nullptr, nullptr, 0);
     recs[c].x = r.x;
     recs[c].n = r.n;
     recs[c].st = r.st;
@@ -603,7 +608,7 @@ __device__ __forceinline__ uint32_t wg_min(uint32_t x, uint32_t* red) {
 __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict__ src,
                                                       const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                       const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                      WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
+                                                      uint64_t chunk, WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
                                                       uint32_t* __restrict__ nblocks, int32_t* __restrict__ status) {
   __shared__ uint4 W[kWin16];
   __shared__ uint64_t xs[kVer];
@@ -662,7 +667,7 @@ __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict_
       E = fb == 0 ? E : xs[fb - 1];
       k += fb;
       if (wid == 0) {
-        const WalkRes w = walk_chunk(W, src, so, Ls, k, E, nullptr, nullptr, 0);
+        const WalkRes w = walk_chunk(W, src, so, Ls, chunk, k, E, nullptr, nullptr, 0);
         if (lane_id() == 0) {
           WalkRec f{E, w.x, w.n, w.st, total, 0u};
           recs[c0 + k] = f;
@@ -695,7 +700,8 @@ __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict_
 __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ src,
                                                   const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                   const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                  const WalkRec* __restrict__ recs, const uint32_t* __restrict__ term,
+                                                  uint64_t chunk, const WalkRec* __restrict__ recs,
+                                                  const uint32_t* __restrict__ term,
                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
                                                   uint64_t* __restrict__ start) {
   const uint64_t total = min(chunk_base[ns], max_chunks);
@@ -709,7 +715,7 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
     if (k > term[s]) continue;
     const WalkRec r = recs[c];
     if (r.n == 0) continue;
-    walk_lane_out(src, so, Ls, r.e, min((k + 1) * (uint64_t)kChunk, Ls), meta, start, blk_off[s] + r.base);
+    walk_lane_out(src, so, Ls, r.e, min((k + 1) * chunk, Ls), meta, start, blk_off[s] + r.base);
   }
 }
 
@@ -736,8 +742,20 @@ __global__ __launch_bounds__(256) void k_walk_serial(const uint8_t* __restrict__
 
 using namespace utl;
 
+// Chunk length of the long-stream walk for a walk over total_bytes. Every
+// chunk costs one entry vote (about 6 ns of the whole device at full
+// occupancy) and the walk of a chunk takes about 2.6 ns per byte (one
+// dependent header load per block at 234-byte blocks), so the sum for B
+// bytes is least near sqrt(2.35 B). The length stays within
+// [kChunk, 4 kChunk]: longer chunks of small blocks make long chains.
+static uint64_t walk_chunk_len(uint64_t total_bytes) {
+  const uint64_t c = (uint64_t)std::sqrt(2.35 * (double)total_bytes);
+  return std::min(std::max(c, (uint64_t)kChunk), (uint64_t)4 * kChunk) & ~15ull;
+}
+
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
-// records, term (one word per stream).
+// records (at most one per kChunk bytes plus one per long stream), term
+// (one word per stream).
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
@@ -750,6 +768,8 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
                                  uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream) {
   if (nstreams == 0) return hipSuccess;
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
+  const uint64_t chunk = walk_chunk_len(total_bytes);
+  const uint64_t est_chunks = std::min(total_bytes / chunk + total_bytes / kLong + 1, max_chunks);  // for grids
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   uint8_t* t = static_cast<uint8_t*>(tmp);
   uint64_t* chunk_base = reinterpret_cast<uint64_t*>(t);
@@ -760,25 +780,27 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   t += max_chunks * sizeof(WalkRec);
   uint32_t* term = reinterpret_cast<uint32_t*>(t);
   const bool chunked = total_bytes >= kLong;
-  const unsigned sgrid = (unsigned)min(nstreams, (uint64_t)16384);
+  const unsigned sgrid = (unsigned)std::min(nstreams, (uint64_t)16384);
   if (chunked) {
-    k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(stream_off, nstreams, chunk_base, tile_sum);
+    k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(stream_off, nstreams, chunk, chunk_base,
+                                                                     tile_sum);
     if (ntiles > 1) {
       k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(tile_sum, ntiles);
       k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
     }
-    const unsigned cgrid = (unsigned)min(max_chunks, (uint64_t)16384);
-    k_walk_vote<<<dim3(min(cgrid, 4096u)), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base,
-                                                                    max_chunks, recs);
-    const unsigned lgrid = (unsigned)min((max_chunks + 63) / 64, (uint64_t)4096);  // one lane per chunk
-    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs);
-    k_walk_verify<<<dim3(min(sgrid, 4096u)), dim3(kVer), 0, stream>>>(src, stream_off, nstreams, chunk_base,
-                                                                       max_chunks, recs, term, nblocks, status);
+    const unsigned vgrid = (unsigned)std::min(est_chunks, (uint64_t)4096);
+    k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+                                                       recs);
+    const unsigned lgrid = (unsigned)std::min((est_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
+    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+                                                        recs);
+    k_walk_verify<<<dim3(std::min(sgrid, 4096u)), dim3(kVer), 0, stream>>>(
+        src, stream_off, nstreams, chunk_base, max_chunks, chunk, recs, term, nblocks, status);
     if (meta)
-      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, recs,
-                                                         term, blk_off, meta, start);
+      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+                                                        recs, term, blk_off, meta, start);
   }
-  const unsigned lsgrid = (unsigned)min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
+  const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
   k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
                                                          max_chunks, blk_off, meta, start, nblocks, status);
   return hipGetLastError();
