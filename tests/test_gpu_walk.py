@@ -199,17 +199,27 @@ def test_walk_long_streams_chunked(ctx, orc):
 def test_walk_many_streams_long_in_every_tile(ctx, orc):
     """More streams than one tile of the chunk-offset scan (1024 streams):
     long streams in the first, middle and last tiles, the last stream long,
-    between short and empty ones, so that every tile offset is used."""
+    between short and empty ones, so that every tile offset is used. Two of
+    them have more than 64 chunks (the wide verification), one of those a
+    20000-byte zero run in the middle (5000 valid 4-byte blocks, whose
+    entry votes can miss and are walked again)."""
     rng = np.random.default_rng(26)
     streams, _ = make_streams(orc, rng, 3000, 200, False)
-    for i in (3, 1023, 1024, 1500, 2047, 2999):
+    for i in (3, 1023, 1024, 2047):
         n = int(rng.integers(40000, 120000))
         streams[i] = orc.pack_stream(int(rng.choice([0, 1, 2])),
                                      np.cumsum(rng.integers(0, 1 << 8, n)).astype(np.uint32))[0].tobytes()
+    streams[2999] = orc.pack_stream(0, rng.integers(0, 1 << 16, 600000, dtype=np.uint64).astype(np.uint32))[0].tobytes()
+    big = orc.pack_stream(2, rng.integers(0, 1 << 16, 600000, dtype=np.uint64).astype(np.uint32))[0].tobytes()
+    st0 = _starts(orc, big)
+    cut = st0[len(st0) // 2]
+    streams[1500] = big[:cut] + bytes(20000) + big[cut:]
     say(f"walk many: {len(streams)} streams, {sum(map(len, streams))} bytes")
     counts, status = run_walk(ctx, orc, streams, False)
     assert np.all(status == 0)
     assert sum(len(b) >= 32768 for b in streams) == 6
+    assert min(len(streams[1500]), len(streams[2999])) > 65 * 16320
+    assert counts[1500] == len(st0) + 5000
 
 
 def test_walk_large_input_longer_chunks(ctx, orc):

@@ -22,8 +22,10 @@
 //   k_walk_chunks  one lane per chunk walks it from the guess, reading the
 //                  headers from global memory: block count, exit (first
 //                  position at or past the chunk end) and stop status;
-//   k_walk_verify  one workgroup per stream checks, 1024 chunks at a time,
-//                  that each chunk's guess equals the previous chunk's exit;
+//   k_walk_verify  one workgroup per long stream (64 threads for streams of
+//                  at most 64 chunks, else 1024) checks, a workgroup's
+//                  width of chunks at a time, that each chunk's guess
+//                  equals the previous chunk's exit;
 //                  the first chunk that does not is walked again from that
 //                  exit. The result is exact whatever the guesses were: a
 //                  chunk counts only once its entry is the true one, and the
@@ -54,6 +56,7 @@ constexpr int kWarm16 = (kWarm + 64) / 16;  // LDS window of one vote
 constexpr uint64_t kNone = ~0ull;
 constexpr uint16_t kDead = 0xFFFF;
 constexpr uint32_t kStLand = 0, kStEnd = 1, kStStop = 0x100;  // chunk exit kinds (kStStop | error)
+constexpr int kSmallVer = 64;  // long streams of at most this many chunks are verified by 64 threads
 
 struct WalkRec {
   uint64_t e;     // entry (stream-relative), kNone if the vote found none
@@ -385,7 +388,8 @@ constexpr int kPlan = 1024;
 // entry ns closes the last stream); tile_sum[t] = tile t's total.
 __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                      uint64_t chunk, uint64_t* __restrict__ chunk_base,
-                                                     uint64_t* __restrict__ tile_sum) {
+                                                     uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ list,
+                                                     uint32_t* __restrict__ cnt) {
   __shared__ uint64_t red[kPlan / 64];
   const int tid = threadIdx.x, wid = tid >> 6;
   const uint64_t s = (uint64_t)blockIdx.x * kPlan + (uint64_t)tid;
@@ -403,7 +407,22 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict_
   }
   if (s <= ns) chunk_base[s] = before + incl - c;
   if (tid == 0) tile_sum[blockIdx.x] = sum;
+  // the long streams by verification class: list[0 ..) those of at most
+  // kSmallVer chunks, list[ns - 1] downwards the others (one atomic per
+  // wavefront and class)
+  const int ln = lane_id();
+  const uint64_t lt = (1ull << ln) - 1;
+  const uint64_t bs = __ballot(c != 0 && c <= (uint64_t)kSmallVer), bb = __ballot(c > (uint64_t)kSmallVer);
+  uint32_t a = 0, b = 0;
+  if (ln == 0 && bs) a = atomicAdd(&cnt[0], (uint32_t)__popcll(bs));
+  if (ln == 0 && bb) b = atomicAdd(&cnt[1], (uint32_t)__popcll(bb));
+  a = (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
+  b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+  if ((bs >> ln) & 1) list[a + (uint32_t)__popcll(bs & lt)] = (uint32_t)s;
+  if ((bb >> ln) & 1) list[ns - 1 - (b + (uint32_t)__popcll(bb & lt))] = (uint32_t)s;
 }
+
+__global__ void k_walk_zero(uint32_t* cnt) { cnt[threadIdx.x] = 0u; }
 
 // Exclusive scan of q[0..n) in place (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
@@ -589,27 +608,33 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
   }
 }
 
-// Verification and block numbering of every long stream (one workgroup of
-// 1024 threads per stream, 1024 chunks per round). term[s] is the index of
-// the stream's last live chunk.
-constexpr int kVer = 1024;
-
-// Minimum and sum over the workgroup (16 wavefronts); red holds 32 words.
+// Verification and block numbering of the long streams (one workgroup of
+// NT threads per stream, NT chunks per round): streams of at most kSmallVer
+// chunks by 64-thread workgroups, the others by 1024-thread ones, each
+// class from its list (written by k_walk_plan). term[s] is the index of the
+// stream's last live chunk.
+// Minimum over the workgroup (NT / 64 wavefronts); red holds 32 words.
+template <int NT>
 __device__ __forceinline__ uint32_t wg_min(uint32_t x, uint32_t* red) {
   x = wave_reduce_min(x);
+  if (NT == 64) return x;
   if (lane_id() == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
   uint32_t m = red[0];
-  for (int w = 1; w < kVer / 64; ++w) m = min(m, red[w]);
+  for (int w = 1; w < NT / 64; ++w) m = min(m, red[w]);
   __syncthreads();
   return m;
 }
 
-__global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict__ src,
-                                                      const uint64_t* __restrict__ stream_off, uint64_t ns,
-                                                      const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                      uint64_t chunk, WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
-                                                      uint32_t* __restrict__ nblocks, int32_t* __restrict__ status) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ src,
+                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
+                                                    const uint64_t* __restrict__ chunk_base, uint64_t chunk,
+                                                    const uint32_t* __restrict__ list, const uint32_t* __restrict__ cnt,
+                                                    WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
+                                                    uint32_t* __restrict__ nblocks, int32_t* __restrict__ status) {
+  constexpr int kVer = NT;
+  constexpr bool kSmall = NT == kSmallVer;
   __shared__ uint4 W[kWin16];
   __shared__ uint64_t xs[kVer];
   __shared__ uint32_t sts[kVer];
@@ -617,10 +642,11 @@ __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict_
   __shared__ uint64_t fix_x;
   __shared__ uint32_t fix_n, fix_st;
   const int tid = threadIdx.x, wid = tid >> 6;
-  for (uint64_t s = blockIdx.x; s < ns; s += gridDim.x) {
+  const uint32_t nlist = cnt[kSmall ? 0 : 1];
+  for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+    const uint64_t s = kSmall ? list[i] : list[ns - 1 - i];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
-    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
     const uint64_t c0 = chunk_base[s];
     const uint64_t nch = chunk_base[s + 1] - c0;
     uint64_t E = 0;  // true entry of chunk k
@@ -635,8 +661,8 @@ __global__ __launch_bounds__(kVer) void k_walk_verify(const uint8_t* __restrict_
       sts[tid] = r.st;
       __syncthreads();
       const uint64_t prev = tid == 0 ? E : xs[tid - 1];
-      const uint32_t fb = wg_min(in && r.e != prev ? (uint32_t)tid : (uint32_t)kVer, red);
-      const uint32_t ft = wg_min(in && r.st != kStLand ? (uint32_t)tid : (uint32_t)kVer, red);
+      const uint32_t fb = wg_min<NT>(in && r.e != prev ? (uint32_t)tid : (uint32_t)kVer, red);
+      const uint32_t ft = wg_min<NT>(in && r.st != kStLand ? (uint32_t)tid : (uint32_t)kVer, red);
       const uint32_t ngood = fb <= ft ? fb : ft + 1;  // chunks k .. k + ngood - 1 are verified
       const uint32_t cnt = (uint32_t)tid < ngood && in ? r.n : 0u;
       const uint32_t incl = wave_incl_scan(cnt);
@@ -754,36 +780,43 @@ static uint64_t walk_chunk_len(uint64_t total_bytes) {
 }
 
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
-// records (at most one per kChunk bytes plus one per long stream), term
-// (one word per stream).
+// records (at most one per kChunk bytes plus one per long stream), term and
+// the long-stream lists (one word per stream each), two list counters.
+static uint64_t walk_words(uint64_t n, uint64_t bytes) { return (n * bytes + 255) / 256 * 256; }
+
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
-  return ((nstreams + 1) * 8 + 255) / 256 * 256 + (ntiles * 8 + 255) / 256 * 256 + max_chunks * sizeof(WalkRec) +
-         nstreams * 4 + 256;
+  return walk_words(nstreams + 1, 8) + walk_words(ntiles, 8) + max_chunks * sizeof(WalkRec) +
+         2 * walk_words(nstreams, 4) + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
                                  uint64_t total_bytes, const uint64_t* blk_off, uint32_t* meta, uint64_t* start,
                                  uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream) {
   if (nstreams == 0) return hipSuccess;
+  if (nstreams > 0xFFFFFFFFull) return hipErrorInvalidValue;  // stream numbers are 32-bit words
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t chunk = walk_chunk_len(total_bytes);
   const uint64_t est_chunks = std::min(total_bytes / chunk + total_bytes / kLong + 1, max_chunks);  // for grids
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   uint8_t* t = static_cast<uint8_t*>(tmp);
   uint64_t* chunk_base = reinterpret_cast<uint64_t*>(t);
-  t += ((nstreams + 1) * 8 + 255) / 256 * 256;
+  t += walk_words(nstreams + 1, 8);
   uint64_t* tile_sum = reinterpret_cast<uint64_t*>(t);
-  t += (ntiles * 8 + 255) / 256 * 256;
+  t += walk_words(ntiles, 8);
   WalkRec* recs = reinterpret_cast<WalkRec*>(t);
   t += max_chunks * sizeof(WalkRec);
   uint32_t* term = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(nstreams, 4);
+  uint32_t* list = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(nstreams, 4);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(t);
   const bool chunked = total_bytes >= kLong;
-  const unsigned sgrid = (unsigned)std::min(nstreams, (uint64_t)16384);
   if (chunked) {
+    k_walk_zero<<<dim3(1), dim3(2), 0, stream>>>(cnt);
     k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(stream_off, nstreams, chunk, chunk_base,
-                                                                     tile_sum);
+                                                                     tile_sum, list, cnt);
     if (ntiles > 1) {
       k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(tile_sum, ntiles);
       k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
@@ -794,8 +827,13 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
     const unsigned lgrid = (unsigned)std::min((est_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
     k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
                                                         recs);
-    k_walk_verify<<<dim3(std::min(sgrid, 4096u)), dim3(kVer), 0, stream>>>(
-        src, stream_off, nstreams, chunk_base, max_chunks, chunk, recs, term, nblocks, status);
+    // grids: every workgroup resident at once (a stream's verification is
+    // a few rounds of loads and barriers)
+    const uint64_t nlong_max = total_bytes / kLong + 1;
+    k_walk_verify<kSmallVer><<<dim3((unsigned)std::min(nlong_max, (uint64_t)8192)), dim3(kSmallVer), 0, stream>>>(
+        src, stream_off, nstreams, chunk_base, chunk, list, cnt, recs, term, nblocks, status);
+    k_walk_verify<1024><<<dim3((unsigned)std::min(nlong_max, (uint64_t)512)), dim3(1024), 0, stream>>>(
+        src, stream_off, nstreams, chunk_base, chunk, list, cnt, recs, term, nblocks, status);
     if (meta)
       k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
                                                         recs, term, blk_off, meta, start);
