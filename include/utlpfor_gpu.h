@@ -153,8 +153,11 @@ int utl_block_lengths(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_bloc
 
 /* Index of foreign streams whose block starts are unknown: the reference's
  * serial walk (block k+1 starts BlockLength(block k) bytes after block k,
- * block_length.go:8, example_test.go:171-187) on the device, one wavefront
- * per stream. Stream s is d_src[d_stream_off[s] : d_stream_off[s+1]]. Its
+ * block_length.go:8, example_test.go:171-187) on the device, a lane per
+ * short stream and chunk-parallel for long ones (an entry vote per chunk,
+ * verified against the previous chunk's exit, so the result is the serial
+ * walk's for any input). Stream s is d_src[d_stream_off[s] :
+ * d_stream_off[s+1]]. At most 2^32 - 1 streams. Its
  * walk stops at the stream's end (d_status[s] = 0), at the first BlockLength
  * error (that error), or at a block longer than the bytes left
  * (ErrInvalidBuffer); d_nblocks[s] counts the blocks before the stop. With

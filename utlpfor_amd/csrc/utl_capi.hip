@@ -519,6 +519,10 @@ int utl_index_walk(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_stream_
     g_last_error = "utl_index_walk: d_src must be 16-byte aligned";
     return UTL_ERR_ARGUMENT;
   }
+  if (nstreams > 0xFFFFFFFFull) {  // stream numbers are 32-bit words on the device
+    g_last_error = "utl_index_walk: more than 2^32 - 1 streams";
+    return UTL_ERR_ARGUMENT;
+  }
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
