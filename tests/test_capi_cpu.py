@@ -1,7 +1,9 @@
 """CPU-only checks of the C-ABI library: it loads, exports every symbol the
 header declares, and its host-side entry points (Header, BlockLength,
 MaxBlockLength: a few-byte header read each, no device work) agree with the
-oracle. No compute entry point is called without a GPU."""
+oracle. No compute entry point is called without a GPU: the one device
+entry point called here returns on its argument checks, before any device
+work."""
 import os
 import re
 import struct
@@ -69,6 +71,19 @@ def test_host_entry_points_match_oracle(lib, orc):
             wl, we = orc.BlockLength(b)
             gl, ge = codec.BlockLength(b)
             assert (ge.code if ge else 0) == we and (gl == wl or we)
+
+
+def test_walk_rejects_bad_arguments_before_device_work(lib):
+    """utl_index_walk checks its arguments before it touches a device: a
+    missing required pointer, block starts without meta words, an unaligned
+    source and more than 2^32 - 1 streams are UTL_ERR_ARGUMENT. The fake
+    pointers are never dereferenced."""
+    fake = 0x100000  # 16-byte aligned
+    err = 100  # UTL_ERR_ARGUMENT
+    assert lib.utl_index_walk(None, None, fake, 1, None, None, None, fake, fake) == err
+    assert lib.utl_index_walk(None, fake, fake, 1, None, None, fake, fake, fake) == err
+    assert lib.utl_index_walk(None, fake + 4, fake, 1, None, None, None, fake, fake) == err
+    assert lib.utl_index_walk(None, fake, fake, 1 << 32, None, None, None, fake, fake) == err
 
 
 def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
