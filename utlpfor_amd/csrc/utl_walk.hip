@@ -215,9 +215,9 @@ __device__ WalkRes walk_range(const uint32_t* W32, uint64_t wabs, uint64_t so, u
   return {pos + off, n, st};
 }
 
-// Walks chunk k of a long stream from pos (a wave-collective call).
 // Walks from pos while pos < end through kStep-byte LDS windows (a
-// wave-collective call).
+// wave-collective call; the verification's re-walk of a chunk whose entry
+// vote missed).
 __device__ WalkRes walk_span(uint4* W, const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos,
                              uint64_t end, uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
   uint32_t n = 0;
