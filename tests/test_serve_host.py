@@ -26,11 +26,14 @@ def _engine(cap, clock, tick_s):
     eng.heap = type("H", (), {"cap": cap})()
     eng.batches = []
 
-    def run_batch(reqs, deliver=True):
+    def launch(reqs, half):  # one tick's kernels: no deliveries
         eng.batches.append([id(r) for r in reqs])
+        eng.halves.append(half)
         clock.t += tick_s
+        return []
 
-    eng.run_batch = run_batch
+    eng.halves = []
+    eng._launch = launch
     return eng
 
 
@@ -40,12 +43,28 @@ def test_admission_fifo_within_heap(monkeypatch):
     reqs = [serve.Request(serve.DECOMPRESS, 0.0, stream=0, first=i, nblocks=10) for i in range(6)]
     cost = serve.Engine.cost(_engine(1, clock, 0), reqs[0])
     eng = _engine(cost * 2, clock, 0.01)
-    stats = eng.serve(reqs, clock=clock)
+    # without delivery a request is done when its tick's kernels are
+    stats = eng.serve(reqs, deliver=False, clock=clock)
     assert [len(b) for b in eng.batches] == [2, 2, 2]
     assert eng.batches[0] == [id(reqs[0]), id(reqs[1])]  # arrival order
+    assert eng.halves == [0, 1, 0]  # ticks use the heap halves in turn
     lat = [r.latency() for r in reqs]
     assert np.allclose(lat, [0.01, 0.01, 0.02, 0.02, 0.03, 0.03])
     assert stats["ticks"] == 3 and stats["requests"] == 6
+
+
+def test_delivery_marks_requests_done(monkeypatch):
+    """With delivery, each tick's results are delivered by a worker thread
+    (here nothing to copy) and every request gets a completion time no
+    earlier than its tick's kernels, in tick order."""
+    clock = _Clock()
+    monkeypatch.setattr(serve.time, "sleep", lambda s: setattr(clock, "t", clock.t + s))
+    reqs = [serve.Request(serve.DECOMPRESS, 0.0, stream=0, first=i, nblocks=10) for i in range(6)]
+    cost = serve.Engine.cost(_engine(1, clock, 0), reqs[0])
+    eng = _engine(cost * 2, clock, 0.01)
+    stats = eng.serve(reqs, clock=clock)
+    assert stats["ticks"] == 3 and [r.tick for r in reqs] == [0, 0, 1, 1, 2, 2]
+    assert all(r.latency() >= 0.01 * (r.tick + 1) - 1e-12 for r in reqs)
 
 
 def test_late_arrivals_join_next_tick(monkeypatch):
@@ -53,7 +72,7 @@ def test_late_arrivals_join_next_tick(monkeypatch):
     monkeypatch.setattr(serve.time, "sleep", lambda s: setattr(clock, "t", clock.t + s))
     eng = _engine(1 << 40, clock, 0.005)
     reqs = [serve.Request(serve.GET, t, stream=0, positions=np.array([1, 2])) for t in (0.0, 0.001, 0.004, 0.02)]
-    eng.serve(reqs, clock=clock)
+    eng.serve(reqs, deliver=False, clock=clock)
     # the first tick starts at t=0 with one request; the two that arrive
     # during it share the next tick; the last arrives after an idle gap
     assert [len(b) for b in eng.batches] == [1, 2, 1]

@@ -23,6 +23,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.lib = _lib.load()
+        self.device = device
         self.handle = self.lib.utl_ctx_create(device)
         if not self.handle:
             raise UtlRuntimeError(f"no usable gfx950 device {device}: {_lib.last_error()}")

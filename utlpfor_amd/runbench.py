@@ -165,7 +165,7 @@ def serving(ctx: Context, store_bytes: int = 16 << 30, rate: float = 2000.0, nre
         eng = serve.Engine(ctx, store, heap_fraction=0.25)
         eng.run_batch(reqs[:8], deliver)  # warm-up tick
         stats = eng.serve(reqs, deliver=deliver)
-        stats["heap_bytes"] = eng.heap.cap
+        stats["heap_bytes"] = sum(h.cap for h in eng.heaps)
         if deliver:
             stats["bit_exact_sampled"] = _check_served(ctx, store, reqs[::check_every])
         out["to_host" if deliver else "in_hbm"] = stats

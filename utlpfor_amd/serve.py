@@ -214,7 +214,12 @@ def _values_of(st: StoreStream, first: int, nblocks: int) -> int:
 
 
 class Engine:
-    """Continuous-batching executor over a Store (one HIP stream)."""
+    """Continuous-batching executor over a Store. Kernels run on ctx's
+    stream; results go to the host through a second context (its own
+    stream and pinned buffer), so that serve() delivers a tick's results
+    while the next tick's kernels run. The heap is two halves that ticks
+    use in turn: a half is reused only once its tick's results are
+    delivered."""
 
     def __init__(self, ctx: Context, store: Store, heap_bytes: int | None = None, heap_fraction: float = 0.5):
         if heap_bytes is None:
@@ -222,13 +227,20 @@ class Engine:
             heap_bytes = int(free * heap_fraction)
         self.ctx, self.store = ctx, store
         self.lib = _lib.load()
-        self.heap = _Heap(ctx, heap_bytes)
-        self.stage = _HostStage(ctx)
+        half = heap_bytes // 2
+        self.heaps = [_Heap(ctx, half), _Heap(ctx, half)]
+        self.heap = self.heaps[0]
+        self.stage = _HostStage(ctx)  # uploads, on ctx's stream
+        self.dctx = Context(getattr(ctx, "device", 0))
+        self.dstage = _HostStage(self.dctx)  # deliveries, on the second stream
 
     def close(self) -> None:
-        """Frees the heap and the host staging buffer."""
+        """Frees the heaps and the host staging buffers."""
         self.stage.close()
-        self.heap.mem.free()
+        self.dstage.close()
+        self.dctx.close()
+        for h in self.heaps:
+            h.mem.free()
 
     # ---- admission ----
     def cost(self, r: Request) -> int:
@@ -263,7 +275,17 @@ class Engine:
     def run_batch(self, reqs: list[Request], deliver: bool = True) -> None:
         """Runs one batch to completion (results copied to the host when
         deliver is set)."""
-        lib, ctx, store, heap = self.lib, self.ctx, self.store, self.heap
+        pending = self._launch(reqs, 0)
+        if deliver:
+            for fn in pending:
+                fn()
+
+    def _launch(self, reqs: list[Request], half: int) -> list:
+        """Launches one batch on heap half `half` and waits for its kernels;
+        returns the functions that deliver its results (they use only the
+        delivery context)."""
+        ctx, store = self.ctx, self.store
+        self.heap = heap = self.heaps[half]
         heap.reset()
         pending = []  # (request, kind-specific handles) to deliver after the sync
         for is64 in (False, True):
@@ -277,9 +299,7 @@ class Engine:
             if comp:
                 pending += self._compress(comp, is64)
         ctx.sync()
-        if deliver:
-            for fn in pending:
-                fn()
+        return pending
 
     def _h2d(self, a: np.ndarray) -> int:
         a = np.ascontiguousarray(a)
@@ -289,9 +309,10 @@ class Engine:
         return p
 
     def _d2h(self, p: int, n: int, dtype) -> np.ndarray:
+        """Device to host through the delivery context."""
         out = np.empty(n, dtype=dtype)
         if out.nbytes:
-            check(self.lib.utl_memcpy_d2h(self.ctx.handle, out.ctypes.data, p, out.nbytes), "d2h")
+            check(self.lib.utl_memcpy_d2h(self.dctx.handle, out.ctypes.data, p, out.nbytes), "d2h")
         return out
 
     def _decompress(self, reqs: list[Request], is64: bool):
@@ -322,8 +343,8 @@ class Engine:
             for r, (u0, nv, e0, nb) in zip(reqs, spans):
                 r.result = np.empty(nv, dt)
                 r.status = np.empty(nb, np.int32)
-            self.stage.fetch(out, u * item, [(u0 * item, r.result) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
-            self.stage.fetch(status, 4 * max(e - 1, 0), [(4 * e0, r.status) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
+            self.dstage.fetch(out, u * item, [(u0 * item, r.result) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
+            self.dstage.fetch(status, 4 * max(e - 1, 0), [(4 * e0, r.status) for r, (u0, nv, e0, nb) in zip(reqs, spans)])
         return [deliver]
 
     def _get(self, reqs: list[Request], is64: bool):
@@ -349,8 +370,8 @@ class Engine:
             for r, (a, k) in zip(reqs, spans):
                 r.result = np.empty(k, dt)
                 r.status = np.empty(k, np.int32)
-            self.stage.fetch(out, item * n, [(item * a, r.result) for r, (a, k) in zip(reqs, spans)])
-            self.stage.fetch(status, 4 * n, [(4 * a, r.status) for r, (a, k) in zip(reqs, spans)])
+            self.dstage.fetch(out, item * n, [(item * a, r.result) for r, (a, k) in zip(reqs, spans)])
+            self.dstage.fetch(status, 4 * n, [(4 * a, r.status) for r, (a, k) in zip(reqs, spans)])
         return [deliver]
 
     def _compress(self, reqs: list[Request], is64: bool):
@@ -391,7 +412,7 @@ class Engine:
                 r.result = np.empty(int(bo[b1]) - int(bo[b0]), np.uint8)
                 r.block_len = (metas[b0:b1] & 0xFFFFFF).astype(np.int64)
                 dsts.append((int(bo[b0]), r.result))
-            self.stage.fetch(data, int(bo[-1]), dsts)
+            self.dstage.fetch(data, int(bo[-1]), dsts)
         return [deliver]
 
     # ---- the loop ----
@@ -404,6 +425,16 @@ class Engine:
         t0 = clock()
         i = tick = 0
         sizes = []
+        worker = ThreadPoolExecutor(max_workers=1) if deliver else None
+        inflight = [None, None]  # delivery of the last tick on each heap half
+
+        def finish(pending, b, tick):
+            for fn in pending:
+                fn()
+            t = clock() - t0
+            for r in b:
+                r.done, r.tick = t, tick
+
         while i < len(reqs):
             now = clock() - t0
             if reqs[i].arrival > now:
@@ -417,12 +448,21 @@ class Engine:
                 b.append(reqs[i])
                 used += c
                 i += 1
-            self.run_batch(b, deliver)
-            t = clock() - t0
-            for r in b:
-                r.done, r.tick = t, tick
+            half = tick & 1
+            if inflight[half] is not None:
+                inflight[half].result()  # the half's results are on the host: reuse it
+            pending = self._launch(b, half)
+            if worker is not None:
+                inflight[half] = worker.submit(finish, pending, b, tick)
+            else:
+                finish([], b, tick)
             tick += 1
             sizes.append(len(b))
+        for f in inflight:
+            if f is not None:
+                f.result()
+        if worker is not None:
+            worker.shutdown()
         wall = clock() - t0
         return summarize(reqs, wall, self.store, ticks=tick, mean_batch=float(np.mean(sizes)) if sizes else 0.0)
 
