@@ -177,6 +177,21 @@ int utl_index_walk(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_stream_
                    const uint64_t *d_blk_off, uint32_t *d_meta, uint64_t *d_block_start, uint32_t *d_nblocks,
                    int32_t *d_status);
 
+/* Wave index of a stream described by its meta words: d_wave[2w] and
+ * d_wave[2w+1] are the byte and value offsets of block 8w, for w <
+ * ceil(nblocks / 8) (the same device prefix scan of block lengths as
+ * utl_index_build, kept at one entry per eight blocks: 2 bytes per block
+ * instead of 16). d_meta and d_wave must be 16-byte aligned. */
+int utl_wave_index_build(utl_ctx *ctx, const uint32_t *d_meta, size_t nblocks, uint64_t *d_wave);
+
+/* Batched UnpackUint32 / UnpackUint64 (is_u64) of concatenated blocks in
+ * order, found from their meta words and the wave index (each wavefront of
+ * the decode scans its eight meta words itself): results identical to
+ * utl_decode_u32/u64 over the index utl_index_build makes from the same meta
+ * words. persistent selects the work-queue form. */
+int utl_decode_meta(utl_ctx *ctx, int is_u64, int persistent, const uint8_t *d_src, const uint32_t *d_meta,
+                    const uint64_t *d_wave, size_t nblocks, void *d_out, int32_t *d_status, int32_t *d_consumed);
+
 /* Batched UnpackUint32 over an index (one launch). d_status[k] receives the
  * block's status word; d_consumed (optional, may be NULL) its consumed
  * bytes. Block k's values go to d_out[d_out_off[k] ..]; values of failed
@@ -191,17 +206,36 @@ int utl_decode_u64(utl_ctx *ctx, const uint8_t *d_src, const uint64_t *d_block_o
                    size_t nblocks, uint64_t *d_out, int32_t *d_status, int32_t *d_consumed);
 
 /* Persistent form of the batched decode (is_u64 selects UnpackUint64): a
- * fixed grid of wavefronts pulls runs of blocks from a device work queue.
- * Launch shape from utl_set_tune (chosen by the autotuner in
- * utlpfor_amd/autotune.py); results identical to utl_decode_u32/u64. On
- * gfx950 it is slower than the flat launch (a looping wavefront waits for
- * its previous stores with every load): use it where a fixed grid is
- * wanted. */
+ * fixed grid of workgroups pulls runs of tiles from a device work queue.
+ * Results identical to utl_decode_u32/u64. On gfx950 it is slower than the
+ * flat launch (a looping wavefront waits for its previous stores with every
+ * load): use it where a fixed grid is wanted. */
 int utl_decode_persistent(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
                           const uint64_t *d_out_off, size_t nblocks, void *d_out, int32_t *d_status,
                           int32_t *d_consumed);
-int utl_set_tune(utl_ctx *ctx, int waves_per_wg, int grid_wgs, int blocks_per_wave);
-int utl_get_tune(utl_ctx *ctx, int *waves_per_wg, int *grid_wgs, int *blocks_per_wave);
+
+/* Launch variant of the batched decode kernels on a context (no reference
+ * counterpart; the reference tunes its own path constants offline,
+ * internal/threshold/main.go:36). Chosen per workload by the autotuner
+ * (utlpfor_amd/autotune.py); every variant gives identical results.
+ *   arena_bytes    LDS bytes a wavefront stages its eight uint32 blocks in:
+ *                  2560, 3328 or 4352. Larger arenas take fewer staging
+ *                  rounds on streams of wide blocks, at fewer waves per CU.
+ *                  uint64 blocks always use an 8320-byte arena.
+ *   waves_per_wg   wavefronts per workgroup, 2 or 4 (8 blocks each).
+ *   grid_wgs       persistent form only: workgroups in the grid.
+ *   tiles_per_grab persistent form only: tiles of 8 * waves_per_wg blocks a
+ *                  workgroup takes per work-queue grab.
+ * The flat forms (utl_decode_u32/u64, graphs) use arena_bytes and
+ * waves_per_wg; the per-call entry points always use the 2560-byte arena. */
+typedef struct utl_tune_params {
+  int arena_bytes;
+  int waves_per_wg;
+  int grid_wgs;
+  int tiles_per_grab;
+} utl_tune_params;
+int utl_set_tune(utl_ctx *ctx, const utl_tune_params *t);
+int utl_get_tune(utl_ctx *ctx, utl_tune_params *t);
 
 /* Batched GetUint32 / GetUint64 (get.go:74, uint64.go:364): request i reads
  * position d_req_pos[i] of block d_req_block[i]; with d_req_block NULL,
@@ -250,9 +284,15 @@ int utl_make_jobs(utl_ctx *ctx, const uint64_t *d_val_off, const uint64_t *d_blk
 
 /* One resident stream's whole decode -- the device index scan over d_meta
  * and the batched decode (persistent work-queue form when `persistent`, with
- * the context's current launch shape) -- captured once as a HIP graph and
- * replayed by utl_graph_launch on the context's stream. All pointers are
- * fixed at capture. Returns NULL on failure (utl_last_error). */
+ * the context's current launch variant) -- captured once as a HIP graph and
+ * replayed by utl_graph_launch on the context's stream. With d_block_off and
+ * d_out_off NULL the graph keeps only the wave index (utl_wave_index_build,
+ * in memory it owns) and decodes in meta form (utl_decode_meta); otherwise
+ * it writes the full index there and decodes from it. All pointers are
+ * fixed at capture. The graph owns the scratch of its index scan, so other
+ * calls on the same context never disturb a replay; the persistent form
+ * uses the context's work counters, so a graph must not outlive its
+ * context. Returns NULL on failure (utl_last_error). */
 typedef struct utl_graph utl_graph;
 utl_graph *utl_graph_decode_create(utl_ctx *ctx, int is_u64, int persistent, const uint8_t *d_src,
                                    const uint32_t *d_meta, size_t nblocks, uint64_t *d_block_off,

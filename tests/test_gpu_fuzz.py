@@ -143,6 +143,72 @@ def test_fuzz_gathered_damaged(ctx, orc, seed, is64):
     _case(ctx, orc, 1000 + seed, is64, persistent=bool(seed % 2))
 
 
+# every launch variant of the decode (utl_set_tune): LDS arena x waves per
+# workgroup, flat and persistent (uint64 blocks have one arena size)
+VARIANTS = [(False, a, w) for a in (2560, 3328, 4352) for w in (2, 4)] + [(True, 2560, w) for w in (2, 4)]
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("is64,arena,waves", VARIANTS)
+def test_fuzz_every_launch_variant(ctx, orc, is64, arena, waves, persistent):
+    from utlpfor_amd import batch
+    before = batch.get_tune(ctx)
+    _, cus = ctx.device_info()
+    batch.set_tune(ctx, arena, waves, cus * 2, 3)
+    say(f"launch variant: arena {arena}, {waves} waves per workgroup, persistent={persistent}")
+    try:
+        _case(ctx, orc, 2000 + arena + 10 * waves + int(persistent), is64, persistent)
+    finally:
+        batch.set_tune(ctx, before["arena_bytes"], before["waves_per_wg"], before["grid_wgs"],
+                       before["tiles_per_grab"])
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("is64,arena,waves", VARIANTS)
+def test_fuzz_meta_form(ctx, orc, is64, arena, waves, persistent):
+    """The meta form (utl_decode_meta over the wave index) on a damaged
+    store, every launch variant: statuses, overflow positions and values
+    equal the oracle decoding each block's own bytes, and failed blocks
+    keep the poison."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(3000 + arena + 10 * waves + int(persistent) + 7 * int(is64))
+    cs, data, lens, share = _damaged_store(ctx, rng, is64)
+    before = batch.get_tune(ctx)
+    _, cus = ctx.device_info()
+    batch.set_tune(ctx, arena, waves, cus * 2, 2)
+    say(f"meta form: {'u64' if is64 else 'u32'} arena {arena}, {waves} waves, persistent={persistent}: "
+        f"{cs.nblocks} blocks ({share:.0%} damaged)")
+    try:
+        item = 8 if is64 else 4
+        out = DeviceArray(max(cs.nvalues, 1) * item, np.uint8, ctx)
+        out.fill_bytes(POISON)
+        st = DeviceArray(cs.nblocks, np.int32, ctx)
+        wave = batch.build_wave_index(ctx, cs)
+        batch.decode_meta(ctx, cs, wave, out=out, status=st, persistent=persistent)
+        ctx.sync()
+    finally:
+        batch.set_tune(ctx, before["arena_bytes"], before["waves_per_wg"], before["grid_wgs"],
+                       before["tiles_per_grab"])
+    boff, ooff = cs.block_off.to_numpy(), cs.out_off.to_numpy()
+    wout, wst = orc.unpack_blocks(data, boff, ooff, cs.nvalues, is64)
+    gst = st.to_numpy()
+    wcode = np.where((wst & 0xFF) == 99, 1, wst & 0xFF)
+    assert np.array_equal(gst & 0xFF, wcode)
+    assert np.array_equal(gst >> 8, np.where(wcode == 6, wst >> 8, 0))
+    dt = np.uint64 if is64 else np.uint32
+    got = out.to_numpy()[:cs.nvalues * item].view(dt)
+    want = np.full(cs.nvalues, np.frombuffer(bytes([POISON] * item), dt)[0], dt)
+    for k in np.nonzero(wst == 0)[0]:
+        a = int(ooff[k])
+        c = int(data[int(boff[k])])
+        want[a:a + c] = wout[a:a + c]
+    assert np.array_equal(got, want)
+    for a in (out, st, wave):
+        a.free()
+    cs.free()
+
+
 @pytest.mark.parametrize("seed", range(3))
 @pytest.mark.parametrize("is64", [False, True])
 def test_fuzz_get_damaged(ctx, orc, seed, is64):

@@ -1,8 +1,8 @@
 """Device index scan (utl_index_build) against a host cumulative sum: block
 byte offsets and value offsets from per-block meta words (length | count
 << 24), at sizes around every tile boundary, above the 1024-tile point where
-the tile-prefix pass is added, and from unaligned pointers (the per-item
-fallback kernels)."""
+the group-scan pass is added (several groups of 1024 tiles, the last one
+partial), and from unaligned pointers (the per-item fallback kernels)."""
 import ctypes as C
 import sys
 
@@ -36,8 +36,8 @@ def _meta(rng, n):
     return (lens | (cnts << 24)).astype(np.uint32)
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 1023, 1024, 1025, 4097, 1024 * 1024, 1024 * 1024 + 7,
-                               3 * 1024 * 1024 + 5])
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 1023, 1024, 1025, 4097, 1024 * 1024, 1024 * 1024 + 7, 1025 * 1024,
+                               3 * 1024 * 1024 + 5, 26 * 1024 * 1024 + 333])
 def test_index_build_sizes(ctx, n):
     from utlpfor_amd import batch
     from utlpfor_amd.device import DeviceArray
@@ -67,3 +67,24 @@ def test_index_build_unaligned(ctx, n):
     ctx.sync()
     wb, wo = _want(meta)
     assert np.array_equal(db.to_numpy()[1:], wb) and np.array_equal(do.to_numpy()[1:], wo)
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 1023, 1024, 1025, 4097, 1024 * 1024 + 7, 3 * 1024 * 1024 + 5,
+                               26 * 1024 * 1024 + 333])
+def test_wave_index_build(ctx, n):
+    """utl_wave_index_build: the byte and value offsets of every eighth block."""
+    from utlpfor_amd import _lib
+    from utlpfor_amd.device import DeviceArray, check
+    rng = np.random.default_rng(3 * n + 1)
+    meta = _meta(rng, n)
+    say(f"wave index: {n} blocks")
+    dm = DeviceArray.from_numpy(meta, ctx)
+    nw = (n + 7) // 8
+    dw = DeviceArray(2 * nw + 2, np.uint64, ctx)
+    dw.fill_bytes(0xEE)
+    check(_lib.load().utl_wave_index_build(ctx.handle, dm.ptr, n, dw.ptr), "utl_wave_index_build")
+    ctx.sync()
+    wb, wo = _want(meta)
+    got = dw.to_numpy()
+    assert np.array_equal(got[0:2 * nw:2], wb[0:n:8]) and np.array_equal(got[1:2 * nw:2], wo[0:n:8])
+    assert np.all(got[2 * nw:] == np.uint64(0xEEEEEEEEEEEEEEEE))  # nothing past the last entry

@@ -352,19 +352,36 @@ def test_uneven_streams_graph_autotune(gpu, orc):
     ctx.sync()
     ref = flat_out.to_numpy()
     assert np.all(flat_st.to_numpy() == 0) and np.array_equal(ref, v)
+    # every launch variant decodes the mix exactly, flat and persistent
+    before = batch.get_tune(ctx)
+    vout = DeviceArray(v.size, np.uint32, ctx)
+    vst = DeviceArray(s.nblocks, np.int32, ctx)
+    for c in autotune.candidates(ctx, s)[::7]:
+        batch.set_tune(ctx, c["arena_bytes"], c["waves_per_wg"], c["grid_wgs"], c["tiles_per_grab"])
+        vout.fill_bytes(0xA5)
+        batch.decode(ctx, s, out=vout, status=vst, persistent=c["persistent"])
+        ctx.sync()
+        assert np.all(vst.to_numpy() == 0) and np.array_equal(vout.to_numpy(), v), c
+    vout.free()
+    vst.free()
+    batch.set_tune(ctx, before["arena_bytes"], before["waves_per_wg"], before["grid_wgs"], before["tiles_per_grab"])
     tune = autotune.tune_decode(ctx, s, use_cache=False, reps=2)
     assert tune["grid_wgs"] % 8 == 0 and tune["tiles_per_grab"] >= 1
+    assert tune["arena_bytes"] in batch.ARENAS32 and tune["waves_per_wg"] in batch.WAVES_PER_WG
+    assert tune["ms"] == min(tune["flat_ms"], tune["persistent_ms"])
+    names = {(c["arena_bytes"], c["waves_per_wg"], c["persistent"]) for c in tune["candidates"]}
+    assert names == {(a, w, p) for a in batch.ARENAS32 for w in batch.WAVES_PER_WG for p in (False, True)}
     out = DeviceArray(v.size, np.uint32, ctx)
     st = DeviceArray(s.nblocks, np.int32, ctx)
-    for persistent in (True, False):
-        g = batch.DecodeGraph(ctx, s, out, st, persistent=persistent)
+    for persistent, block_index in ((True, True), (False, True), (True, False), (False, False)):
+        g = batch.DecodeGraph(ctx, s, out, st, persistent=persistent, block_index=block_index)
         for rep in range(3):
             out.fill_bytes(0xA5)
             ctx.sync()
             g.launch()
             ctx.sync()
-            assert np.all(st.to_numpy() == 0), (persistent, rep)
-            assert np.array_equal(out.to_numpy(), ref), (persistent, rep)
+            assert np.all(st.to_numpy() == 0), (persistent, block_index, rep)
+            assert np.array_equal(out.to_numpy(), ref), (persistent, block_index, rep)
             # eager work between replays
             batch.decode(ctx, s, out=flat_out, status=flat_st)
             ctx.sync()

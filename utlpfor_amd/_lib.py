@@ -50,8 +50,10 @@ SIGNATURES = {
     "utl_decode_u32": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, vp]),
     "utl_decode_u64": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, vp]),
     "utl_decode_persistent": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp, vp]),
-    "utl_set_tune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
-    "utl_get_tune": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "utl_wave_index_build": (C.c_int, [vp, vp, sz, vp]),
+    "utl_decode_meta": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, sz, vp, vp, vp]),
+    "utl_set_tune": (C.c_int, [vp, vp]),
+    "utl_get_tune": (C.c_int, [vp, vp]),
     "utl_get_batch": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp, sz, vp, vp]),
     "utl_gather_index": (C.c_int, [vp, vp, vp, sz, vp, sz, sz, vp, vp]),
     "utl_index_walk": (C.c_int, [vp, vp, vp, sz, vp, vp, vp, vp, vp]),

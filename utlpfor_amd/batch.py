@@ -172,6 +172,27 @@ def decode(ctx: Context, s: DeviceStream, out: DeviceArray | None = None, status
     return out, status
 
 
+def build_wave_index(ctx: Context, s: DeviceStream, wave: DeviceArray | None = None) -> DeviceArray:
+    """The wave index of s (byte and value offsets of every eighth block)."""
+    if wave is None:
+        wave = DeviceArray(2 * ((s.nblocks + 7) // 8) + 2, np.uint64, ctx)
+    check(_lib.load().utl_wave_index_build(ctx.handle, s.meta.ptr, s.nblocks, wave.ptr), "utl_wave_index_build")
+    return wave
+
+
+def decode_meta(ctx: Context, s: DeviceStream, wave: DeviceArray, out: DeviceArray | None = None,
+                status: DeviceArray | None = None, consumed: DeviceArray | None = None, persistent: bool = False):
+    """One launch decoding every block of s from its meta words and wave index."""
+    if out is None:
+        out = DeviceArray(max(s.nvalues, 1), np.uint64 if s.is64 else np.uint32, ctx)
+    if status is None:
+        status = DeviceArray(max(s.nblocks, 1), np.int32, ctx)
+    cptr = consumed.ptr if consumed is not None else None
+    check(_lib.load().utl_decode_meta(ctx.handle, int(s.is64), int(persistent), s.data.ptr, s.meta.ptr, wave.ptr,
+                                      s.nblocks, out.ptr, status.ptr, cptr), "utl_decode_meta")
+    return out, status
+
+
 def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: DeviceArray | None = None,
               status: DeviceArray | None = None, blocks: DeviceArray | None = None):
     """Batched Get of stream positions (or of (block, in-block pos) pairs)."""
@@ -189,14 +210,19 @@ def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: Device
 
 class DecodeGraph:
     """The whole decode of a resident stream (device index scan + decode)
-    captured once as a HIP graph and replayed with one launch."""
+    captured once as a HIP graph and replayed with one launch. With
+    block_index False the graph keeps only the wave index (one entry per
+    eight blocks) and decodes from the meta words; otherwise it rewrites the
+    stream's block_off / out_off and decodes from them."""
 
     def __init__(self, ctx: Context, s: DeviceStream, out: DeviceArray, status: DeviceArray,
-                 persistent: bool = True):
+                 persistent: bool = True, block_index: bool = True):
         lib = _lib.load()
         self.ctx = ctx
+        bo = s.block_off.ptr if block_index else None
+        oo = s.out_off.ptr if block_index else None
         self.handle = lib.utl_graph_decode_create(ctx.handle, int(s.is64), int(persistent), s.data.ptr, s.meta.ptr,
-                                                  s.nblocks, s.block_off.ptr, s.out_off.ptr, out.ptr, status.ptr)
+                                                  s.nblocks, bo, oo, out.ptr, status.ptr)
         if not self.handle:
             raise RuntimeError(f"graph capture failed: {_lib.last_error()}")
 
@@ -215,14 +241,26 @@ class DecodeGraph:
             pass
 
 
-def set_tune(ctx: Context, waves_per_wg: int, grid_wgs: int, blocks_per_wave: int) -> None:
-    check(_lib.load().utl_set_tune(ctx.handle, waves_per_wg, grid_wgs, blocks_per_wave), "utl_set_tune")
+class TuneParams(C.Structure):
+    """utl_tune_params of the C-ABI: the decode launch variant."""
+    _fields_ = [("arena_bytes", C.c_int), ("waves_per_wg", C.c_int), ("grid_wgs", C.c_int),
+                ("tiles_per_grab", C.c_int)]
 
 
-def get_tune(ctx: Context):
-    a, b, c = C.c_int(), C.c_int(), C.c_int()
-    check(_lib.load().utl_get_tune(ctx.handle, C.byref(a), C.byref(b), C.byref(c)), "utl_get_tune")
-    return a.value, b.value, c.value
+ARENAS32 = (2560, 3328, 4352)
+WAVES_PER_WG = (2, 4)
+
+
+def set_tune(ctx: Context, arena_bytes: int, waves_per_wg: int, grid_wgs: int, tiles_per_grab: int) -> None:
+    t = TuneParams(arena_bytes, waves_per_wg, grid_wgs, tiles_per_grab)
+    check(_lib.load().utl_set_tune(ctx.handle, C.byref(t)), "utl_set_tune")
+
+
+def get_tune(ctx: Context) -> dict:
+    t = TuneParams()
+    check(_lib.load().utl_get_tune(ctx.handle, C.byref(t)), "utl_get_tune")
+    return {"arena_bytes": t.arena_bytes, "waves_per_wg": t.waves_per_wg, "grid_wgs": t.grid_wgs,
+            "tiles_per_grab": t.tiles_per_grab}
 
 
 # ---- stream container (utl_container_* in the C-ABI) ----

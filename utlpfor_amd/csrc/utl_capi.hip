@@ -28,6 +28,8 @@ namespace {
 constexpr size_t kReadWindow = 72 * 1024;
 constexpr size_t kStageBytes = 128 * 1024;
 constexpr size_t kResultOff = 96 * 1024;  // results region inside the staging buffers
+// Launch variant of the per-call decode (one block): the smallest LDS arena.
+const utl_tune kOneBlockTune{2560, 4, 1, 1};
 
 thread_local std::string g_last_error;
 
@@ -54,7 +56,7 @@ struct utl_ctx {
   void* d_tmp = nullptr;
   size_t d_tmp_bytes = 0;
   unsigned long long* d_counter = nullptr;
-  utl_tune tune{4, 0, 4};
+  utl_tune tune{2560, 4, 0, 4};  // small arena, 4 waves per workgroup (the tuner may change it)
   int cu_count = 0;
   char name[128] = {0};
 };
@@ -250,7 +252,7 @@ static int unpack_call(utl_ctx* ctx, bool u64, const uint8_t* src, size_t len, v
   int32_t* d_cons = d_status + 1;
   void* d_vals = d + kResultOff + 64;
   HIP_TRY(utl_launch_decode(u64, d + 64, (const uint64_t*)d, (const uint64_t*)(d + 16), 1, d_vals, d_status, d_cons,
-                            c->stream));
+                            kOneBlockTune, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int32_t st = *(volatile int32_t*)(c->h_stage + kResultOff);
   const int code = UTL_STATUS_CODE(st);
@@ -462,7 +464,8 @@ int utl_decode_u32(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_block_o
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_decode(false, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, d_consumed, c->stream));
+  HIP_TRY(utl_launch_decode(false, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, d_consumed, c->tune,
+                            c->stream));
   return UTL_OK;
 }
 
@@ -472,7 +475,8 @@ int utl_decode_u64(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_block_o
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_decode(true, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, d_consumed, c->stream));
+  HIP_TRY(utl_launch_decode(true, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, d_consumed, c->tune,
+                            c->stream));
   return UTL_OK;
 }
 
@@ -488,24 +492,55 @@ int utl_decode_persistent(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const 
   return UTL_OK;
 }
 
-int utl_set_tune(utl_ctx* ctx, int waves_per_wg, int grid_wgs, int blocks_per_wave) {
+int utl_wave_index_build(utl_ctx* ctx, const uint32_t* d_meta, size_t nblocks, uint64_t* d_wave) {
+  if (nblocks && (!d_meta || !d_wave)) return UTL_ERR_ARGUMENT;
+  if ((((uintptr_t)d_meta | (uintptr_t)d_wave) & 15) != 0) {
+    g_last_error = "utl_wave_index_build: d_meta and d_wave must be 16-byte aligned";
+    return UTL_ERR_ARGUMENT;
+  }
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  if (waves_per_wg < 1 || waves_per_wg > 4 || grid_wgs < 1 || blocks_per_wave < 1) return UTL_ERR_ARGUMENT;
-  c->tune.waves_per_wg = waves_per_wg;
-  c->tune.grid_wgs = grid_wgs;
-  c->tune.blocks_per_wave = (uint32_t)blocks_per_wave;
+  if ((err = ensure_tmp(c, utl_index_tmp_bytes(nblocks)))) return err;
+  HIP_TRY(utl_launch_wave_index(d_meta, nblocks, d_wave, c->d_tmp, c->stream));
   return UTL_OK;
 }
 
-int utl_get_tune(utl_ctx* ctx, int* waves_per_wg, int* grid_wgs, int* blocks_per_wave) {
+int utl_decode_meta(utl_ctx* ctx, int is_u64, int persistent, const uint8_t* d_src, const uint32_t* d_meta,
+                    const uint64_t* d_wave, size_t nblocks, void* d_out, int32_t* d_status, int32_t* d_consumed) {
+  if (nblocks && (!d_src || !d_meta || !d_wave || !d_out || !d_status)) return UTL_ERR_ARGUMENT;
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  if (waves_per_wg) *waves_per_wg = c->tune.waves_per_wg;
-  if (grid_wgs) *grid_wgs = c->tune.grid_wgs;
-  if (blocks_per_wave) *blocks_per_wave = (int)c->tune.blocks_per_wave;
+  HIP_TRY(utl_launch_decode_meta(is_u64 != 0, d_src, d_meta, d_wave, nblocks, d_out, d_status, d_consumed,
+                                 persistent ? c->d_counter : nullptr, c->tune, c->stream));
+  return UTL_OK;
+}
+
+int utl_set_tune(utl_ctx* ctx, const utl_tune_params* t) {
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  if (!t) return UTL_ERR_ARGUMENT;
+  const utl_tune n{t->arena_bytes, t->waves_per_wg, t->grid_wgs, (uint32_t)(t->tiles_per_grab > 0 ? t->tiles_per_grab : 0)};
+  if (!utl_tune_valid(n)) {
+    g_last_error = "utl_set_tune: arena_bytes must be 2560, 3328 or 4352, waves_per_wg 2 or 4, grid_wgs and "
+                   "tiles_per_grab >= 1";
+    return UTL_ERR_ARGUMENT;
+  }
+  c->tune = n;
+  return UTL_OK;
+}
+
+int utl_get_tune(utl_ctx* ctx, utl_tune_params* t) {
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  if (!t) return UTL_ERR_ARGUMENT;
+  t->arena_bytes = c->tune.arena_bytes;
+  t->waves_per_wg = c->tune.waves_per_wg;
+  t->grid_wgs = c->tune.grid_wgs;
+  t->tiles_per_grab = (int)c->tune.tiles_per_grab;
   return UTL_OK;
 }
 
@@ -605,10 +640,15 @@ int utl_make_jobs(utl_ctx* ctx, const uint64_t* d_val_off, const uint64_t* d_blk
 
 }  // extern "C"
 
+// A graph owns the scratch its captured index scan writes (tile totals), so
+// later calls on the same context that grow or free the context's own
+// scratch never touch memory a live graph replays into.
 struct utl_graph {
   utl_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  void* d_tmp = nullptr;
+  uint64_t* d_wave = nullptr;  // the wave index, when the caller asks for no block index
 };
 
 extern "C" {
@@ -618,22 +658,41 @@ utl_graph* utl_graph_decode_create(utl_ctx* ctx, int is_u64, int persistent, con
                                    uint64_t* d_out_off, void* d_out, int32_t* d_status) {
   int err;
   utl_ctx* c = resolve(ctx, &err);
-  if (!c || !d_meta || !d_block_off || !d_out_off || (nblocks && (!d_src || !d_out || !d_status))) return nullptr;
+  const bool wave_form = !d_block_off && !d_out_off;
+  if (!c || !d_meta || (!wave_form && (!d_block_off || !d_out_off)) || (nblocks && (!d_src || !d_out || !d_status)))
+    return nullptr;
+  if (wave_form && ((uintptr_t)d_meta & 15) != 0) {
+    g_last_error = "utl_graph_decode_create: d_meta must be 16-byte aligned without a block index";
+    return nullptr;
+  }
   // everything the captured kernels touch must exist before capture
-  if (ensure_tmp(c, utl_index_tmp_bytes(nblocks))) return nullptr;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
   utl_graph* g = new utl_graph();
   g->ctx = c;
-  bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  bool ok = hipMalloc(&g->d_tmp, utl_index_tmp_bytes(nblocks)) == hipSuccess;
+  if (ok && wave_form)
+    ok = hipMalloc((void**)&g->d_wave, (utl_wave_index_entries(nblocks) + 2) * sizeof(uint64_t)) == hipSuccess;
+  if (!ok) {
+    g_last_error = "graph scratch allocation failed";
+    hipFree(g->d_tmp);
+    delete g;
+    return nullptr;
+  }
+  unsigned long long* counter = persistent ? c->d_counter : nullptr;
+  ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
   if (ok) {
-    ok = utl_launch_index(d_meta, nblocks, d_block_off, d_out_off, c->d_tmp, c->stream) == hipSuccess;
-    if (ok) {
-      if (persistent)
+    if (wave_form) {
+      ok = utl_launch_wave_index(d_meta, nblocks, g->d_wave, g->d_tmp, c->stream) == hipSuccess &&
+           utl_launch_decode_meta(is_u64 != 0, d_src, d_meta, g->d_wave, nblocks, d_out, d_status, nullptr, counter,
+                                  c->tune, c->stream) == hipSuccess;
+    } else {
+      ok = utl_launch_index(d_meta, nblocks, d_block_off, d_out_off, g->d_tmp, c->stream) == hipSuccess;
+      if (ok && persistent)
         ok = utl_launch_decode_persistent(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_out, d_status,
-                                          nullptr, c->d_counter, c->tune, c->stream) == hipSuccess;
-      else
+                                          nullptr, counter, c->tune, c->stream) == hipSuccess;
+      else if (ok)
         ok = utl_launch_decode(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_out, d_status, nullptr,
-                               c->stream) == hipSuccess;
+                               c->tune, c->stream) == hipSuccess;
     }
     hipGraph_t graph = nullptr;
     const bool ended = hipStreamEndCapture(c->stream, &graph) == hipSuccess;
@@ -645,6 +704,8 @@ utl_graph* utl_graph_decode_create(utl_ctx* ctx, int is_u64, int persistent, con
     g_last_error = "graph capture failed";
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    hipFree(g->d_tmp);
+    hipFree(g->d_wave);
     delete g;
     return nullptr;
   }
@@ -664,6 +725,8 @@ void utl_graph_destroy(utl_graph* g) {
   hipStreamSynchronize(g->ctx->stream);
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
+  if (g->d_tmp) hipFree(g->d_tmp);
+  if (g->d_wave) hipFree(g->d_wave);
   delete g;
 }
 

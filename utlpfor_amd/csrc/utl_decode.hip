@@ -28,17 +28,22 @@ namespace utl {
 
 constexpr int kTpb = 8;             // threads per block
 constexpr int kWaveBlocks = 8;      // blocks per wavefront
-constexpr int kWgWaves = 4;         // wavefronts per workgroup
-constexpr int kTileBlocks = kWaveBlocks * kWgWaves;
 // Bytes of a sub-block the decoder can read: header, svbLen, FOR64 base,
 // block2Len (16) + payload (512) + bitmap (16) + control (32) + data (512).
 constexpr int kSubExtent = 1088;
-// Staged bytes per wavefront: u32 blocks use 2560 (>= 15 + 2176 + 4); u64
-// blocks 8320, which holds all eight blocks of a wave in one round even when
-// they are two-block blocks of bw 32 (7 * 1034 + 1034 + 19 bytes), at three
-// workgroups per CU (one fewer than the registers allow; two staging round
-// trips per wave cost more: 8Mi uniform two-block 30.5 -> 24.7 us).
-constexpr int kArena32 = 2560;
+// Staged bytes per wavefront (the arena) and wavefronts per workgroup are
+// launch variants (template parameters), chosen per workload by the
+// autotuner. u32 arenas: 2560 B (24 waves per CU by LDS; a wave whose eight
+// blocks span more takes further staging rounds, each a round trip to HBM),
+// 3328 B (20 or 22 waves per CU) and 4352 B (16 or 18), which holds eight
+// blocks of bit width 32 (8 x 528 + 19 bytes) in one round. u64 blocks use
+// 8320 B, which holds all eight blocks of a wave in one round even when they
+// are two-block blocks of bw 32 (7 * 1034 + 1034 + 19 bytes), at three
+// workgroups per CU (two staging round trips per wave cost more: 8Mi uniform
+// two-block 30.5 -> 24.7 us).
+constexpr int kArena32Small = 2560;
+constexpr int kArena32Mid = 3328;
+constexpr int kArena32Large = 4352;
 constexpr int kArena64 = 8320;
 // Exceptions of the wave's blocks are numbered flat: block b's control bytes
 // are control bytes [cbase_b, cbase_b + ctrl_b) of the wave, with cbase_b the
@@ -67,19 +72,13 @@ struct alignas(16) WaveLdsT {
   uint32_t arena[ARENA / 4];
   uint32_t aux[kAux];  // exception high bits by flat slot (and the table entries before that)
 };
-// u32: 4 x 6672 B per workgroup, 6 workgroups per CU; u64: 4 x 12432 B, 3
-// workgroups per CU.
-static_assert(sizeof(WaveLdsT<kArena32>) == 6672, "LDS per wavefront");
-static_assert(6 * 4 * sizeof(WaveLdsT<kArena32>) + 8 <= 160 * 1024, "six u32 workgroups per CU");
+// LDS per wavefront: arena + 4112 B of exception slots. u32 with the small
+// arena: 6672 B, six 4-wave workgroups per CU; u64: 12432 B, three.
+static_assert(sizeof(WaveLdsT<kArena32Small>) == 6672, "LDS per wavefront");
+static_assert(6 * 4 * sizeof(WaveLdsT<kArena32Small>) + 8 <= 160 * 1024, "six u32 workgroups per CU");
+static_assert(5 * 4 * sizeof(WaveLdsT<kArena32Mid>) + 8 <= 160 * 1024, "five u32 workgroups per CU");
+static_assert(9 * 2 * sizeof(WaveLdsT<kArena32Large>) + 8 <= 160 * 1024, "nine 2-wave u32 workgroups per CU");
 static_assert(3 * 4 * sizeof(WaveLdsT<kArena64>) + 8 <= 160 * 1024, "three u64 workgroups per CU");
-template <bool U64>
-struct WaveLdsSel {
-  using type = WaveLdsT<kArena32>;
-};
-template <>
-struct WaveLdsSel<true> {
-  using type = WaveLdsT<kArena64>;
-};
 
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -549,16 +548,20 @@ __device__ __forceinline__ void stage_range(WL* W, const uint8_t* __restrict__ g
 }
 
 // Decoded values are written once and not read again by this kernel. A
-// full block at an even value offset writes whole 64-byte row segments, which
-// go out with non-temporal stores: they stream to HBM instead of allocating
-// L2 lines (on the headline stream this took the kernel from 98 to 72 us).
-// Every other block (partial, or at an odd offset, as in streams of arbitrary
-// lengths) writes 4-byte pieces, which must be merged into lines in L2 first:
-// non-temporal ones would go to HBM as partial lines.
+// full block at a value offset that is a multiple of 16 writes whole 64-byte
+// segments, one per row, which go out with non-temporal stores: they stream
+// to HBM instead of allocating L2 lines (on the 64 Mi delta stream this took
+// the kernel from 98 to 72 us). Any other block (partial, or at an offset
+// that splits its rows across segments, as in streams of arbitrary lengths)
+// goes through the wave's staged store below when it can; here it writes
+// 4-byte pieces, which must be merged into lines in L2 first: non-temporal
+// ones would reach HBM as partial lines.
+__device__ __forceinline__ bool direct_store(int count, uint64_t out0) { return count == kBlock && (out0 & 15) == 0; }
+
 __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t out0, int count, int j,
                                           const uint32_t* v0, const uint32_t* v1) {
   uint32_t* o = out + out0 + 2 * j;
-  if (count == kBlock && (out0 & 1) == 0) {
+  if (direct_store(count, out0)) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r)
       __builtin_nontemporal_store(((unsigned long long)v1[r] << 32) | v0[r],
@@ -692,19 +695,47 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Decodes the wave's blocks [b0, b0 + nb). The wave-uniform round logic
-// stages as many consecutive blocks as fit the arena, decodes them, and
-// repeats until every block is done.
-template <bool U64>
-__device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, const uint8_t* __restrict__ src,
-                                             const uint64_t* __restrict__ boff,
-                             const uint64_t* __restrict__ ooff, uint64_t b0, int nb, void* __restrict__ out,
-                             int32_t* __restrict__ status, int32_t* __restrict__ consumed_out) {
+// Where a launch finds its blocks. Block form: block k is src[boff[k] :
+// boff[k+1]] and decodes to out[ooff[k] ..] (the index scan's output, or a
+// gathered index). Meta form (META): blocks are concatenated in order, block
+// k's meta word (byte length | count << 24) is meta[k], and wave[2w],
+// wave[2w+1] are the byte and value offsets of block 8w (one prefix per
+// wavefront's group of eight blocks, from utl_launch_wave_index); each wave
+// scans its eight meta words itself.
+struct IndexRef {
+  const uint64_t* boff;
+  const uint64_t* ooff;
+  const uint32_t* meta;
+  const uint64_t* wave;
+};
+
+// Decodes the wave's blocks [b0, b0 + nb) (b0 a multiple of 8). The
+// wave-uniform round logic stages as many consecutive blocks as fit the
+// arena, decodes them, and repeats until every block is done.
+template <bool U64, bool META, class WL>
+__device__ __forceinline__ void decode_group(WL* W, const uint8_t* __restrict__ src, const IndexRef ix, uint64_t b0,
+                                             int nb, void* __restrict__ out, int32_t* __restrict__ status,
+                                             int32_t* __restrict__ consumed_out) {
   const int blk = lane_id() >> 3, j = lane_id() & 7;
   const bool mine = blk < nb;
   const uint64_t b = b0 + (mine ? blk : 0);
-  const uint64_t my_o0 = boff[b], my_o1 = boff[b + 1];
-  const uint64_t out0 = ooff[b], slot_n = ooff[b + 1] - out0;
+  // this lane's block: bytes [my_o0, my_o1) of src, slot_n values at out0
+  uint64_t my_o0, my_o1, out0, slot_n;
+  if (META) {
+    const uint32_t m = mine ? ix.meta[b] : 0u;
+    const uint32_t ln = m & 0xFFFFFFu, cn = m >> 24;
+    const uint32_t il = wave_group_scan(ln), ic = wave_group_scan(cn);  // inclusive over the 8 blocks
+    const uint64_t w = b0 >> 3;
+    my_o0 = ix.wave[2 * w] + (il - ln);
+    my_o1 = my_o0 + ln;
+    out0 = ix.wave[2 * w + 1] + (ic - cn);
+    slot_n = cn;
+  } else {
+    my_o0 = ix.boff[b];
+    my_o1 = ix.boff[b + 1];
+    out0 = ix.ooff[b];
+    slot_n = ix.ooff[b + 1] - out0;
+  }
   const int64_t len = (int64_t)(my_o1 - my_o0);
   // Common case: the whole group fits the arena in one round: every block's
   // offsets are monotone (one ballot over the lanes' own offsets) and the
@@ -714,7 +745,7 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
   const uint64_t o_first = readlane64(my_o0, 0);
   const uint64_t o_lastm1 = readlane64(my_o0, last_lane);
   const int64_t ext_all = clamp_extent<U64>((int64_t)(readlane64(my_o1, last_lane) - o_lastm1));
-  constexpr int kArena = WaveLdsSel<U64>::type::kArenaBytes;
+  constexpr int kArena = WL::kArenaBytes;
   const bool one_round = mono && (int64_t)(o_lastm1 - o_first) + ext_all + 15 + 4 <= kArena;
   int first = 0;
 #pragma unroll 1
@@ -731,12 +762,12 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
       // starts need not increase (a gathered index jumps between requests),
       // so the round stages up to the furthest extent among them, not the
       // last one's.
-      base = boff[b0 + first];
+      base = readlane64(my_o0, kTpb * first);
       last = first;
       span = 0;
 #pragma unroll 1
       while (last < nb) {
-        const uint64_t o0 = boff[b0 + last], o1 = boff[b0 + last + 1];
+        const uint64_t o0 = readlane64(my_o0, kTpb * last), o1 = readlane64(my_o1, kTpb * last);
         const int64_t ext = clamp_extent<U64>((int64_t)(o1 - o0));
         if (o0 < base || (int64_t)(o0 - base) + ext + 15 + 4 > kArena) break;
         span = max(span, (int64_t)(o0 - base) + ext);
@@ -744,7 +775,7 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
       }
       if (last == first) {  // (block `first` always fits: its extent is at most kExtent64)
         last = first + 1;
-        span = clamp_extent<U64>((int64_t)(boff[b0 + first + 1] - base));
+        span = clamp_extent<U64>((int64_t)(readlane64(my_o1, kTpb * first) - base));
       }
     }
     const uint8_t* gbase = src + base;
@@ -769,10 +800,14 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
           if (consumed_out) consumed_out[b] = st == UTL_OK ? consumed : 0;
         }
       }
-      // Full blocks at even offsets write whole segments by themselves; a
-      // round with any other block goes through aux when its blocks fill
-      // one range without holes.
-      const bool direct = r1.count == kBlock && (out0 & 1) == 0;
+      // Full blocks at 16-value offsets write whole segments by themselves;
+      // a round with any other block goes through aux when its blocks fill
+      // one range without holes. (Full blocks at other even offsets used to
+      // store their 64-byte rows directly too, each row split across two
+      // segments: on 200,000 uneven streams that wrote 8.7% more bytes to
+      // HBM than the output holds, and staging them took the decode from
+      // 4.35 to 4.11 ms.)
+      const bool direct = direct_store(r1.count, out0);
       const bool holes = __ballot(active && !(st == UTL_OK && (uint64_t)r1.count == slot_n)) != 0;
       if (!holes && __ballot(put && !direct) != 0) {
         const uint64_t g0 = readlane64(out0, kTpb * first);
@@ -847,9 +882,8 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
           if (pmask == 0) break;
           const int k = __ffsll((long long)pmask) - 1;  // lane of the block's thread 0
           pmask &= pmask - 1;
-          const uint64_t kb = b0 + (k >> 3);
           const int ck = __shfl(c1, k, 64), bk = __shfl(b2len, k, 64);
-          const uint8_t* g = src + boff[kb] + ck;
+          const uint8_t* g = src + readlane64(my_o0, k) + ck;
           stage_range(W, g, clamp_extent<false>(bk));
           go_p = bb == (k >> 3);
           at_p = (int)(reinterpret_cast<uintptr_t>(g) & 15);
@@ -873,33 +907,33 @@ __device__ __forceinline__ void decode_group(typename WaveLdsSel<U64>::type* W, 
 // one contiguous range of tiles.
 __device__ __forceinline__ uint64_t xcd_tile(uint64_t g, uint64_t per_xcd) { return (g & 7) * per_xcd + (g >> 3); }
 
-template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 6) void k_decode(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
-                                                const uint64_t* __restrict__ ooff, uint64_t nblocks,
-                                                void* __restrict__ out, int32_t* __restrict__ status,
-                                                int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
-  __shared__ typename WaveLdsSel<U64>::type lds[kWgWaves];
+// Flat form: one workgroup per tile of 8 * WAVES blocks. OCC is the
+// occupancy hint (waves per SIMD) the variant's LDS allows.
+template <bool U64, bool META, int ARENA, int WAVES, int OCC>
+__global__ __launch_bounds__(64 * WAVES, OCC) void k_decode(const uint8_t* __restrict__ src, const IndexRef ix,
+                                                           uint64_t nblocks, void* __restrict__ out,
+                                                           int32_t* __restrict__ status,
+                                                           int32_t* __restrict__ consumed_out, uint64_t per_xcd) {
+  __shared__ WaveLdsT<ARENA> lds[WAVES];
   const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t tile = xcd_tile(blockIdx.x, per_xcd);
-  const uint64_t b0 = tile * kTileBlocks + (uint64_t)wave * kWaveBlocks;
+  const uint64_t b0 = tile * (kWaveBlocks * WAVES) + (uint64_t)wave * kWaveBlocks;
   if (b0 >= nblocks) return;
   const int nb = (int)min((uint64_t)kWaveBlocks, nblocks - b0);
-  decode_group<U64>(&lds[wave], src, boff, ooff, b0, nb, out, status, consumed_out);
+  decode_group<U64, META>(&lds[wave], src, ix, b0, nb, out, status, consumed_out);
 }
 
 // Persistent form: a fixed grid; workgroups on XCD x take tiles from XCD x's
 // own work counter (counter[8x]) over XCD x's contiguous range of tiles, so
 // load balance stays inside the XCD. Every workgroup reaches the exit test.
-template <bool U64>
-__global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 5) void k_decode_persistent(const uint8_t* __restrict__ src,
-                                                           const uint64_t* __restrict__ boff,
-                                                           const uint64_t* __restrict__ ooff, uint64_t nblocks,
-                                                           void* __restrict__ out, int32_t* __restrict__ status,
-                                                           int32_t* __restrict__ consumed_out,
-                                                           unsigned long long* __restrict__ counter, uint64_t per_xcd,
-                                                           uint32_t tiles_per_grab) {
-  __shared__ typename WaveLdsSel<U64>::type lds[kWgWaves];
+template <bool U64, bool META, int ARENA, int WAVES, int OCC>
+__global__ __launch_bounds__(64 * WAVES, OCC) void k_decode_persistent(
+    const uint8_t* __restrict__ src, const IndexRef ix, uint64_t nblocks, void* __restrict__ out,
+    int32_t* __restrict__ status, int32_t* __restrict__ consumed_out, unsigned long long* __restrict__ counter,
+    uint64_t per_xcd, uint32_t tiles_per_grab) {
+  __shared__ WaveLdsT<ARENA> lds[WAVES];
   __shared__ unsigned long long grab;
+  constexpr uint64_t kTileBlocks = kWaveBlocks * WAVES;
   const uint32_t xcd = blockIdx.x & 7;
   const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t ntiles = (nblocks + kTileBlocks - 1) / kTileBlocks;
@@ -916,7 +950,7 @@ __global__ __launch_bounds__(64 * kWgWaves, U64 ? 4 : 5) void k_decode_persisten
       const uint64_t b0 = tile * kTileBlocks + (uint64_t)wave * kWaveBlocks;
       if (b0 < nblocks) {
         const int nb = (int)min((uint64_t)kWaveBlocks, nblocks - b0);
-        decode_group<U64>(&lds[wave], src, boff, ooff, b0, nb, out, status, consumed_out);
+        decode_group<U64, META>(&lds[wave], src, ix, b0, nb, out, status, consumed_out);
       }
     }
   }
@@ -927,38 +961,100 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint64_t
   if (i < n) p[i] = v;
 }
 
+// One launch of variant (U64, ARENA, WAVES) in either index form: the flat
+// grid, or the persistent grid of grid_wgs workgroups when counter is not
+// null.
+template <bool U64, int ARENA, int WAVES, int OCC, int OCC_P>
+hipError_t launch_variant(const uint8_t* src, const IndexRef& ix, uint64_t nblocks, void* out, int32_t* status,
+                          int32_t* consumed, unsigned long long* counter, int grid_wgs, uint32_t grab,
+                          hipStream_t stream) {
+  const uint64_t ntiles = (nblocks + kWaveBlocks * WAVES - 1) / (kWaveBlocks * WAVES);
+  const uint64_t per_xcd = (ntiles + 7) / 8;
+  const dim3 block(64 * WAVES);
+  const bool meta = ix.boff == nullptr;
+  if (!counter) {
+    const dim3 grid((unsigned)(per_xcd * 8));
+    if (meta)
+      k_decode<U64, true, ARENA, WAVES, OCC><<<grid, block, 0, stream>>>(src, ix, nblocks, out, status, consumed,
+                                                                         per_xcd);
+    else
+      k_decode<U64, false, ARENA, WAVES, OCC><<<grid, block, 0, stream>>>(src, ix, nblocks, out, status, consumed,
+                                                                          per_xcd);
+  } else {
+    k_fill_u64<<<1, 64, 0, stream>>>(counter, 0ull, 64);
+    const dim3 grid((unsigned)((grid_wgs + 7) / 8 * 8));
+    grab = grab ? grab : 1;
+    if (meta)
+      k_decode_persistent<U64, true, ARENA, WAVES, OCC_P><<<grid, block, 0, stream>>>(
+          src, ix, nblocks, out, status, consumed, counter, per_xcd, grab);
+    else
+      k_decode_persistent<U64, false, ARENA, WAVES, OCC_P><<<grid, block, 0, stream>>>(
+          src, ix, nblocks, out, status, consumed, counter, per_xcd, grab);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace utl
 
 using namespace utl;
 
-hipError_t utl_launch_decode(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
-                             void* out, int32_t* status, int32_t* consumed, hipStream_t stream) {
+bool utl_tune_valid(const utl_tune& t) {
+  const bool arena = t.arena_bytes == kArena32Small || t.arena_bytes == kArena32Mid || t.arena_bytes == kArena32Large;
+  return arena && (t.waves_per_wg == 2 || t.waves_per_wg == 4) && t.grid_wgs >= 1 && t.tiles_per_grab >= 1;
+}
+
+// The occupancy hints follow the LDS each variant takes per CU (160 KiB):
+// 2560 B: 24 waves (6 per SIMD; the persistent form holds 5 without spills),
+// 3328 B: 20-22 (5), 4352 B: 16-18 (4); u64: 12 waves, hinted 4.
+static hipError_t launch_any(bool u64, const uint8_t* src, const IndexRef& ix, uint64_t nblocks, void* out,
+                             int32_t* status, int32_t* consumed, unsigned long long* counter, const utl_tune& t,
+                             hipStream_t stream) {
   if (nblocks == 0) return hipSuccess;
-  const uint64_t ntiles = (nblocks + kTileBlocks - 1) / kTileBlocks;
-  const uint64_t per_xcd = (ntiles + 7) / 8;
-  const dim3 grid((unsigned)(per_xcd * 8)), block(64 * kWgWaves);
-  if (u64)
-    k_decode<true><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed, per_xcd);
-  else
-    k_decode<false><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed, per_xcd);
-  return hipGetLastError();
+  const int g = t.grid_wgs;
+  const uint32_t q = t.tiles_per_grab;
+  if (u64) {
+    if (t.waves_per_wg == 2)
+      return launch_variant<true, kArena64, 2, 4, 4>(src, ix, nblocks, out, status, consumed, counter, g, q, stream);
+    return launch_variant<true, kArena64, 4, 4, 4>(src, ix, nblocks, out, status, consumed, counter, g, q, stream);
+  }
+  switch (t.arena_bytes * 8 + t.waves_per_wg) {
+    case kArena32Mid * 8 + 2:
+      return launch_variant<false, kArena32Mid, 2, 5, 5>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                         stream);
+    case kArena32Mid * 8 + 4:
+      return launch_variant<false, kArena32Mid, 4, 5, 5>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                         stream);
+    case kArena32Large * 8 + 2:
+      return launch_variant<false, kArena32Large, 2, 4, 4>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                           stream);
+    case kArena32Large * 8 + 4:
+      return launch_variant<false, kArena32Large, 4, 4, 4>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                           stream);
+    case kArena32Small * 8 + 2:
+      return launch_variant<false, kArena32Small, 2, 6, 5>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                           stream);
+    default:
+      return launch_variant<false, kArena32Small, 4, 6, 5>(src, ix, nblocks, out, status, consumed, counter, g, q,
+                                                           stream);
+  }
+}
+
+hipError_t utl_launch_decode(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
+                             void* out, int32_t* status, int32_t* consumed, const utl_tune& tune, hipStream_t stream) {
+  const IndexRef ix{boff, ooff, nullptr, nullptr};
+  return launch_any(u64, src, ix, nblocks, out, status, consumed, nullptr, tune, stream);
 }
 
 hipError_t utl_launch_decode_persistent(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff,
                                         uint64_t nblocks, void* out, int32_t* status, int32_t* consumed,
                                         unsigned long long* counter, const utl_tune& tune, hipStream_t stream) {
-  if (nblocks == 0) return hipSuccess;
-  k_fill_u64<<<1, 64, 0, stream>>>(counter, 0ull, 64);
-  const uint64_t ntiles = (nblocks + kTileBlocks - 1) / kTileBlocks;
-  const uint64_t per_xcd = (ntiles + 7) / 8;
-  const unsigned grid = (unsigned)((tune.grid_wgs + 7) / 8 * 8);
-  const uint32_t grab = tune.blocks_per_wave ? tune.blocks_per_wave : 1;
-  const dim3 block(64 * kWgWaves);
-  if (u64)
-    k_decode_persistent<true><<<dim3(grid), block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
-                                                                 counter, per_xcd, grab);
-  else
-    k_decode_persistent<false><<<dim3(grid), block, 0, stream>>>(src, boff, ooff, nblocks, out, status, consumed,
-                                                                  counter, per_xcd, grab);
-  return hipGetLastError();
+  const IndexRef ix{boff, ooff, nullptr, nullptr};
+  return launch_any(u64, src, ix, nblocks, out, status, consumed, counter, tune, stream);
+}
+
+hipError_t utl_launch_decode_meta(bool u64, const uint8_t* src, const uint32_t* meta, const uint64_t* wave,
+                                  uint64_t nblocks, void* out, int32_t* status, int32_t* consumed,
+                                  unsigned long long* counter, const utl_tune& tune, hipStream_t stream) {
+  const IndexRef ix{nullptr, nullptr, meta, wave};
+  return launch_any(u64, src, ix, nblocks, out, status, consumed, counter, tune, stream);
 }

@@ -159,12 +159,55 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce4(const uint32_t* _
   }
 }
 
-// INLINE: tile_len/tile_cnt hold tile totals and each workgroup sums those
-// before it; otherwise they hold exclusive tile prefixes (k_scan_tiles).
-template <bool INLINE>
+// Groups of kGroupTiles tiles: k_scan_groups turns the tile totals of each
+// group into exclusive prefixes within the group, in place, and writes the
+// group's total; the apply pass then adds the totals of the groups before
+// its own (at most kInlineTiles of them, summed by the workgroup itself).
+constexpr uint64_t kGroupTiles = kTile4;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_groups(uint64_t* __restrict__ tile_len,
+                                                               uint64_t* __restrict__ tile_cnt, uint64_t ntiles,
+                                                               uint64_t* __restrict__ group_len,
+                                                               uint64_t* __restrict__ group_cnt) {
+  __shared__ uint64_t lds[2][4];
+  const uint64_t i0 = (uint64_t)blockIdx.x * kGroupTiles + 4 * threadIdx.x;
+  uint64_t l[4], c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    l[k] = i0 + k < ntiles ? tile_len[i0 + k] : 0;
+    c[k] = i0 + k < ntiles ? tile_cnt[i0 + k] : 0;
+  }
+  uint64_t tl, tc;
+  uint64_t pl = wg_excl_scan(l[0] + l[1] + l[2] + l[3], lds[0], tl);
+  uint64_t pc = wg_excl_scan(c[0] + c[1] + c[2] + c[3], lds[1], tc);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (i0 + k < ntiles) {
+      tile_len[i0 + k] = pl;
+      tile_cnt[i0 + k] = pc;
+    }
+    pl += l[k];
+    pc += c[k];
+  }
+  if (threadIdx.x == 0) {
+    group_len[blockIdx.x] = tl;
+    group_cnt[blockIdx.x] = tc;
+  }
+}
+
+// MODE 0: tile_len/tile_cnt hold tile totals and each workgroup sums those
+// before it. MODE 1: they hold prefixes within groups of kGroupTiles tiles
+// (k_scan_groups) and each workgroup sums the group totals before its own.
+// MODE 2: as 1, with the group totals already turned into exclusive group
+// prefixes (k_scan_tiles over the groups). WAVE: instead of every block's
+// offsets, write only those of blocks 8w (wave[2w], wave[2w+1]): the wave
+// index a decode in meta form reads.
+template <int MODE, bool WAVE = false>
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __restrict__ meta, uint64_t n,
                                                                const uint64_t* __restrict__ tile_len,
                                                                const uint64_t* __restrict__ tile_cnt,
+                                                               const uint64_t* __restrict__ group_len,
+                                                               const uint64_t* __restrict__ group_cnt,
                                                                uint64_t* __restrict__ boff,
                                                                uint64_t* __restrict__ ooff) {
   __shared__ uint64_t lds[4][4];
@@ -172,7 +215,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __
   uint64_t l[4], c[4];
   load_meta4(meta, n, i0, l, c);
   uint64_t tile_pl, tile_pc;
-  if (INLINE) {
+  if (MODE == 0) {
     uint64_t sl = 0, sc = 0;
     for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) {
       sl += tile_len[t];
@@ -180,9 +223,21 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __
     }
     wg_excl_scan(sl, lds[2], tile_pl);
     wg_excl_scan(sc, lds[3], tile_pc);
+  } else if (MODE == 1) {
+    const uint32_t grp = (uint32_t)(blockIdx.x / kGroupTiles);
+    uint64_t sl = 0, sc = 0;
+    for (uint32_t t = threadIdx.x; t < grp; t += kScanThreads) {
+      sl += group_len[t];
+      sc += group_cnt[t];
+    }
+    wg_excl_scan(sl, lds[2], tile_pl);
+    wg_excl_scan(sc, lds[3], tile_pc);
+    tile_pl += tile_len[blockIdx.x];
+    tile_pc += tile_cnt[blockIdx.x];
   } else {
-    tile_pl = tile_len[blockIdx.x];
-    tile_pc = tile_cnt[blockIdx.x];
+    const uint64_t grp = blockIdx.x / kGroupTiles;
+    tile_pl = tile_len[blockIdx.x] + group_len[grp];
+    tile_pc = tile_cnt[blockIdx.x] + group_cnt[grp];
   }
   uint64_t tl, tc;
   uint64_t pl = wg_excl_scan(l[0] + l[1] + l[2] + l[3], lds[0], tl) + tile_pl;
@@ -194,6 +249,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __
   for (int k = 0; k < 4; ++k) {
     b[k + 1] = b[k] + l[k];
     o[k + 1] = o[k] + c[k];
+  }
+  if (WAVE) {
+    if ((i0 & 7) == 0 && i0 < n) *reinterpret_cast<ulonglong2*>(boff + (i0 >> 2)) = make_ulonglong2(b[0], o[0]);
+    return;
   }
   if (i0 + 4 <= n) {
     *reinterpret_cast<ulonglong2*>(boff + i0) = make_ulonglong2(b[0], b[1]);
@@ -253,9 +312,13 @@ using namespace utl;
 
 size_t utl_index_tmp_bytes(uint64_t nblocks) {
   const uint64_t ntiles = (nblocks + kTile4 - 1) / kTile4;  // the smaller of the two tiles
-  return (size_t)(2 * ntiles * sizeof(uint64_t) + 256);
+  const uint64_t ngroups = (ntiles + kGroupTiles - 1) / kGroupTiles;
+  return (size_t)(2 * (ntiles + ngroups) * sizeof(uint64_t) + 256);
 }
 
+// Passes by size: up to kInlineTiles tiles (1 Mi blocks), reduce + apply;
+// above, reduce + group scan + apply, where beyond kInlineTiles groups (1 Gi
+// blocks) one workgroup first scans the group totals serially.
 hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, uint64_t* ooff, void* tmp,
                             hipStream_t stream) {
   if (n == 0) {
@@ -264,14 +327,23 @@ hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, ui
   }
   if ((((uintptr_t)meta | (uintptr_t)boff | (uintptr_t)ooff) & 15) == 0) {
     const uint64_t nt = (n + kTile4 - 1) / kTile4;
+    const uint64_t ng = (nt + kGroupTiles - 1) / kGroupTiles;
     uint64_t* tl = (uint64_t*)tmp;
     uint64_t* tc = tl + nt;
-    k_scan_reduce4<<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc);
+    uint64_t* gl = tc + nt;
+    uint64_t* gc = gl + ng;
+    const dim3 grid((unsigned)nt), block(kScanThreads);
+    k_scan_reduce4<<<grid, block, 0, stream>>>(meta, n, tl, tc);
     if (nt <= kInlineTiles) {
-      k_scan_apply4<true><<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+      k_scan_apply4<0><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, boff, ooff);
     } else {
-      k_scan_tiles<<<1, kScanThreads, 0, stream>>>(tl, tc, nt);
-      k_scan_apply4<false><<<dim3((unsigned)nt), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+      k_scan_groups<<<dim3((unsigned)ng), block, 0, stream>>>(tl, tc, nt, gl, gc);
+      if (ng <= kInlineTiles) {
+        k_scan_apply4<1><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, boff, ooff);
+      } else {
+        k_scan_tiles<<<1, kScanThreads, 0, stream>>>(gl, gc, ng);
+        k_scan_apply4<2><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, boff, ooff);
+      }
     }
     return hipGetLastError();
   }
@@ -281,6 +353,33 @@ hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, ui
   k_scan_reduce<<<dim3((unsigned)ntiles), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc);
   k_scan_tiles<<<1, kScanThreads, 0, stream>>>(tl, tc, ntiles);
   k_scan_apply<<<dim3((unsigned)ntiles), dim3(kScanThreads), 0, stream>>>(meta, n, tl, tc, boff, ooff);
+  return hipGetLastError();
+}
+
+size_t utl_wave_index_entries(uint64_t nblocks) { return (size_t)(2 * ((nblocks + 7) / 8)); }
+
+hipError_t utl_launch_wave_index(const uint32_t* meta, uint64_t n, uint64_t* wave, void* tmp, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  if ((((uintptr_t)meta | (uintptr_t)wave) & 15) != 0) return hipErrorInvalidValue;
+  const uint64_t nt = (n + kTile4 - 1) / kTile4;
+  const uint64_t ng = (nt + kGroupTiles - 1) / kGroupTiles;
+  uint64_t* tl = (uint64_t*)tmp;
+  uint64_t* tc = tl + nt;
+  uint64_t* gl = tc + nt;
+  uint64_t* gc = gl + ng;
+  const dim3 grid((unsigned)nt), block(kScanThreads);
+  k_scan_reduce4<<<grid, block, 0, stream>>>(meta, n, tl, tc);
+  if (nt <= kInlineTiles) {
+    k_scan_apply4<0, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+  } else {
+    k_scan_groups<<<dim3((unsigned)ng), block, 0, stream>>>(tl, tc, nt, gl, gc);
+    if (ng <= kInlineTiles) {
+      k_scan_apply4<1, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+    } else {
+      k_scan_tiles<<<1, kScanThreads, 0, stream>>>(gl, gc, ng);
+      k_scan_apply4<2, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+    }
+  }
   return hipGetLastError();
 }
 

@@ -4,25 +4,40 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Launch shape of the persistent decode kernel, chosen by the autotuner.
+// Launch variant of the decode kernels, chosen by the autotuner.
 struct utl_tune {
-  int waves_per_wg;       // wavefronts per workgroup (1, 2, 4, 8)
-  int grid_wgs;           // workgroups in the persistent grid
-  uint32_t blocks_per_wave;  // blocks a wave takes from the work queue per grab
+  int arena_bytes;          // LDS staging bytes per wavefront, uint32 blocks: 2560, 3328 or 4352
+  int waves_per_wg;         // wavefronts per workgroup: 2 or 4
+  int grid_wgs;             // persistent form: workgroups in the grid
+  uint32_t tiles_per_grab;  // persistent form: tiles a workgroup takes from the work queue per grab
 };
+bool utl_tune_valid(const utl_tune& t);
 
 hipError_t utl_launch_decode(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
-                             void* out, int32_t* status, int32_t* consumed, hipStream_t stream);
+                             void* out, int32_t* status, int32_t* consumed, const utl_tune& tune, hipStream_t stream);
 
 hipError_t utl_launch_decode_persistent(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff,
                                         uint64_t nblocks, void* out, int32_t* status, int32_t* consumed,
                                         unsigned long long* counter, const utl_tune& tune, hipStream_t stream);
+
+// Decode of concatenated blocks described by their meta words and a wave
+// index (the byte and value offsets of every eighth block).
+hipError_t utl_launch_decode_meta(bool u64, const uint8_t* src, const uint32_t* meta, const uint64_t* wave,
+                                  uint64_t nblocks, void* out, int32_t* status, int32_t* consumed,
+                                  unsigned long long* counter, const utl_tune& tune, hipStream_t stream);
 
 // Exclusive scans of (length, count) meta words into block/value offsets.
 // tmp must hold utl_index_tmp_bytes(nblocks) bytes.
 size_t utl_index_tmp_bytes(uint64_t nblocks);
 hipError_t utl_launch_index(const uint32_t* meta, uint64_t nblocks, uint64_t* boff, uint64_t* ooff, void* tmp,
                             hipStream_t stream);
+
+// The wave index: wave[2w], wave[2w+1] = byte and value offsets of block 8w
+// (utl_wave_index_entries(nblocks) entries); same scratch as the index scan.
+// meta and wave must be 16-byte aligned.
+size_t utl_wave_index_entries(uint64_t nblocks);
+hipError_t utl_launch_wave_index(const uint32_t* meta, uint64_t nblocks, uint64_t* wave, void* tmp,
+                                 hipStream_t stream);
 
 hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, const uint64_t* limit, uint64_t nblocks,
                                     uint32_t* meta, int32_t* status, hipStream_t stream);

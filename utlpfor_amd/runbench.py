@@ -106,10 +106,10 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
     s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
     out = DeviceArray(v.size, np.uint32, ctx)
     st = DeviceArray(s.nblocks, np.int32, ctx)
-    say(f"{s.nblocks} blocks, {s.nbytes} bytes; autotuning the persistent decode")
+    say(f"{s.nblocks} blocks, {s.nbytes} bytes; autotuning the decode launch")
     tune = autotune.tune_decode(ctx, s, out, st, use_cache=False, reps=3)
-    # the graph holds whichever decode form the tuner measured faster
-    persistent = tune["ms"] < tune["flat_ms"]
+    # the graph holds the variant and form the tuner measured fastest
+    persistent = tune["persistent"]
     graph = batch.DecodeGraph(ctx, s, out, st, persistent=persistent)
     ms = _time(ctx, graph.launch, reps)
     flat_ms = _time(ctx, lambda: batch.decode(ctx, s, out=out, status=st), reps)
@@ -130,9 +130,20 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
                          {"bit_exact": ok,
                           "form": f"{'persistent' if persistent else 'flat'} decode + index scan in one hipGraph launch",
                           "flat_decode_ms": round(flat_ms, 5),
-                          "tune": {k: tune[k] for k in ("grid_wgs", "tiles_per_grab", "ms", "flat_ms")},
+                          "tune": tune_summary(tune),
                           "streams": nstreams, "gen_seconds": round(gen_s, 1), "walk": wrec})
     s.free()
+    return rec
+
+
+def tune_summary(tune: dict) -> dict:
+    """The tuner's pick and the best time of each (arena, waves, form)."""
+    best = {}
+    for c in tune.get("candidates", []):
+        k = f"{c['arena_bytes']}B/{c['waves_per_wg']}w/{'persistent' if c['persistent'] else 'flat'}"
+        best[k] = round(min(best.get(k, 1e30), c["ms"]), 5)
+    rec = {k: tune[k] for k in ("arena_bytes", "waves_per_wg", "grid_wgs", "tiles_per_grab", "persistent")}
+    rec.update(ms=round(tune["ms"], 5), variants_ms=best)
     return rec
 
 

@@ -16,6 +16,9 @@ uniform64       uint64 uniform over the full range (two-block encoding).
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 STREAM_FLAGS = np.array([0, 1, 2, 4, 3], dtype=np.uint8)
@@ -37,10 +40,15 @@ def delta_postings(n: int = 64 << 20, seed: int = 2) -> np.ndarray:
     return (np.cumsum(gaps, dtype=np.uint64) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
 
 
-def uneven_streams(nstreams: int = 200_000, max_len: int = 200_000, seed: int = 3, progress=None):
-    """Returns (values uint32, stream lengths, stream flags). Generated in
-    chunks of streams with vectorised numpy (segmented cumulative sums for
-    the random walks)."""
+def uneven_streams(nstreams: int = 200_000, max_len: int = 200_000, seed: int = 3, progress=None,
+                   threads: int | None = None):
+    """Returns (values uint32, stream lengths, stream flags). The per-stream
+    parameters come from one seeded generator; the values are drawn in
+    chunks of consecutive streams (at most 2^23 values, or one stream), each
+    from its own generator spawned from the seed, on a pool of threads
+    (numpy's generators and most array operations release the GIL), with
+    segmented cumulative sums for the random walks. The result depends only
+    on the arguments, not on the thread count."""
     rng = np.random.default_rng(seed)
     lens = np.floor(np.exp(rng.uniform(0, np.log(max_len + 1), nstreams))).astype(np.int64)
     lens = np.clip(lens, 1, max_len)
@@ -51,27 +59,40 @@ def uneven_streams(nstreams: int = 200_000, max_len: int = 200_000, seed: int = 
     ends = np.cumsum(lens)
     total = int(ends[-1])
     out = np.empty(total, dtype=np.uint32)
-    chunk = 1 << 26
+    chunk = 1 << 23
+    bounds = []
     s0 = 0
     while s0 < nstreams:
         a = int(ends[s0 - 1]) if s0 else 0
-        s1 = int(np.searchsorted(ends, a + chunk, side="right"))
-        s1 = max(s1, s0 + 1)
+        s1 = max(int(np.searchsorted(ends, a + chunk, side="right")), s0 + 1)
+        bounds.append((s0, s1))
+        s0 = s1
+    seeds = np.random.SeedSequence(seed).spawn(len(bounds))
+
+    def fill(ci: int) -> int:
+        s0, s1 = bounds[ci]
+        crng = np.random.default_rng(seeds[ci])
+        a = int(ends[s0 - 1]) if s0 else 0
         b = int(ends[s1 - 1])
         L = lens[s0:s1]
-        sid = np.repeat(np.arange(s0, s1), L)
         # uniform in [0, 2^b): the top b bits of a uniform 32-bit draw
-        raw = rng.integers(0, 1 << 32, b - a, dtype=np.uint64)
-        uni = raw >> (np.uint64(32) - bits[sid].astype(np.uint64))
-        steps = rng.integers(-(1 << 10), (1 << 10) + 1, b - a, dtype=np.int32).astype(np.int64)
+        raw = crng.integers(0, 1 << 32, b - a, dtype=np.uint64)
+        raw >>= np.repeat((32 - bits[s0:s1]).astype(np.uint64), L)
+        steps = crng.integers(-(1 << 10), (1 << 10) + 1, b - a, dtype=np.int32).astype(np.int64)
         cs = np.cumsum(steps)
         seg0 = np.concatenate([[0], np.cumsum(L)[:-1]])
-        base = np.repeat(cs[seg0] - steps[seg0], L)  # cumsum before each stream's first step
-        wv = (start[sid] + cs - base).astype(np.int64).astype(np.uint32)
-        out[a:b] = np.where(walk[sid], wv, uni.astype(np.uint32))
-        if progress:
-            progress(f"generated {b}/{total} values")
-        s0 = s1
+        # the walk's value: its start plus the steps since the stream's first
+        cs += np.repeat(start[s0:s1] - (cs[seg0] - steps[seg0]), L)
+        out[a:b] = np.where(np.repeat(walk[s0:s1], L), cs.astype(np.uint32), raw.astype(np.uint32))
+        return b
+
+    nthreads = threads or max(1, min(16, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(nthreads) as ex:
+        done = 0
+        for b in ex.map(fill, range(len(bounds))):
+            done = max(done, b)
+            if progress and done % (1 << 28) < chunk:
+                progress(f"generated {done}/{total} values")
     return out, lens.astype(np.uint64), flags
 
 
