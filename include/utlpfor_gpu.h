@@ -145,6 +145,18 @@ int utl_dev_mem_info(size_t *free_bytes, size_t *total_bytes);
 int utl_index_build(utl_ctx *ctx, const uint32_t *d_meta, size_t nblocks, uint64_t *d_block_off,
                     uint64_t *d_out_off);
 
+/* utl_index_build that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). No counterpart in the reference. The scan's
+ * scratch is a fixed 1 MiB device array per GPU (set up by utl_ctx_create),
+ * so nblocks may be at most 67,026,944 (UTL_ERR_ARGUMENT beyond, with the
+ * reason in utl_last_error); ctx must not be NULL (creating the default
+ * context allocates). The scratch is shared by every call on the device:
+ * calls that overlap on one GPU (other contexts or streams) race. Output is
+ * identical to utl_index_build's. */
+int utl_index_build_noalloc(utl_ctx *ctx, const uint32_t *d_meta, size_t nblocks, uint64_t *d_block_off,
+                            uint64_t *d_out_off);
+
 /* Header-parse pass over a device stream whose block starts are known:
  * BlockLength (block_length.go:8) of block k = src[start[k]:limit[k]], as
  * meta words (length | count << 24) and per-block status. */

@@ -51,6 +51,7 @@ SIGNATURES = {
     "utl_decode_u64": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, vp]),
     "utl_decode_persistent": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp, vp]),
     "utl_wave_index_build": (C.c_int, [vp, vp, sz, vp]),
+    "utl_index_build_noalloc": (C.c_int, [vp, vp, sz, vp, vp]),
     "utl_decode_meta": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, sz, vp, vp, vp]),
     "utl_set_tune": (C.c_int, [vp, vp]),
     "utl_get_tune": (C.c_int, [vp, vp]),

@@ -87,6 +87,7 @@ int ctx_init(utl_ctx* c, int device) {
   HIP_TRY(hipHostMalloc((void**)&c->h_stage, kStageBytes, hipHostMallocMapped | hipHostMallocCoherent));
   HIP_TRY(hipHostGetDevicePointer((void**)&c->d_stage, c->h_stage, 0));
   HIP_TRY(hipMalloc((void**)&c->d_counter, 4096));  // 8 per-XCD work counters, 64 B apart
+  HIP_TRY(utl_index_scratch_init(device));             // the fixed scratch of utl_index_build_noalloc
   c->tune.grid_wgs = c->cu_count * 5;  // five persistent u32 workgroups are resident per CU
   return UTL_OK;
 }
@@ -489,6 +490,31 @@ int utl_decode_persistent(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const 
   if (!c) return err;
   HIP_TRY(utl_launch_decode_persistent(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_out, d_status,
                                        d_consumed, c->d_counter, c->tune, c->stream));
+  return UTL_OK;
+}
+
+int utl_index_build_noalloc(utl_ctx* ctx, const uint32_t* d_meta, size_t nblocks, uint64_t* d_block_off,
+                            uint64_t* d_out_off) {
+  if (!ctx) {
+    g_last_error = "utl_index_build_noalloc: needs a context from utl_ctx_create (creating the default one allocates)";
+    return UTL_ERR_ARGUMENT;
+  }
+  if ((!d_meta && nblocks) || !d_block_off || !d_out_off) return UTL_ERR_ARGUMENT;
+  const size_t need = utl_index_tmp_bytes(nblocks);
+  if (need > utl_index_static_scratch_bytes()) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "utl_index_build_noalloc: %zu blocks need %zu scratch bytes, more than the fixed %zu",
+             nblocks, need, utl_index_static_scratch_bytes());
+    g_last_error = buf;
+    return UTL_ERR_ARGUMENT;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  void* tmp = utl_index_scratch_ptr(ctx->device);
+  if (!tmp) {
+    g_last_error = "utl_index_build_noalloc: the fixed scratch of this device was not set up";
+    return UTL_ERR_HIP;
+  }
+  HIP_TRY(utl_launch_index(d_meta, nblocks, d_block_off, d_out_off, tmp, ctx->stream));
   return UTL_OK;
 }
 

@@ -356,6 +356,38 @@ hipError_t utl_launch_index(const uint32_t* meta, uint64_t n, uint64_t* boff, ui
   return hipGetLastError();
 }
 
+// Scratch of utl_index_build_noalloc: the scan's tile and group totals in a
+// fixed device array (each device holds its own copy until the process
+// ends), for callers that cannot allocate. Its device address is looked up
+// once per device at context creation (utl_index_scratch_init); host code
+// uses that address only, never the array's name.
+namespace utl {
+constexpr size_t kStaticScratchBytes = 1u << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+}  // namespace utl
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+size_t utl_index_static_scratch_bytes() { return kStaticScratchBytes; }
+
+hipError_t utl_index_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (__atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE)) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(utl::scratch));  // the current device's copy
+  if (e != hipSuccess) return e;
+  __atomic_store_n(&g_scratch_ptr[device], p, __ATOMIC_RELEASE);
+  return hipSuccess;
+}
+
+void* utl_index_scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  return __atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE);
+}
+
 size_t utl_wave_index_entries(uint64_t nblocks) { return (size_t)(2 * ((nblocks + 7) / 8)); }
 
 hipError_t utl_launch_wave_index(const uint32_t* meta, uint64_t n, uint64_t* wave, void* tmp, hipStream_t stream) {

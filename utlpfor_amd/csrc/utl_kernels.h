@@ -32,6 +32,13 @@ size_t utl_index_tmp_bytes(uint64_t nblocks);
 hipError_t utl_launch_index(const uint32_t* meta, uint64_t nblocks, uint64_t* boff, uint64_t* ooff, void* tmp,
                             hipStream_t stream);
 
+// The index scan's fixed device scratch (utl_index_build_noalloc): its size,
+// the once-per-device lookup of its address (with that device current), and
+// that address (null before the lookup).
+size_t utl_index_static_scratch_bytes();
+hipError_t utl_index_scratch_init(int device);
+void* utl_index_scratch_ptr(int device);
+
 // The wave index: wave[2w], wave[2w+1] = byte and value offsets of block 8w
 // (utl_wave_index_entries(nblocks) entries); same scratch as the index scan.
 // meta and wave must be 16-byte aligned.
