@@ -94,8 +94,9 @@ int utl_block_length(const uint8_t *src, size_t len, int64_t *out_len);
 int utl_pack_u32(utl_ctx *ctx, uint32_t flag, uint32_t *values, size_t n, uint8_t *dst, size_t dst_cap,
                  size_t *out_len);
 
-/* unpack.go:5 UnpackUint32. Decodes one block on the GPU. dst must hold 128
- * values; *count values are valid. *consumed as the reference returns it.
+/* unpack.go:5 UnpackUint32. Decodes one block on the GPU into dst, which
+ * must hold 128 values or at least the block's count (src[0]): only the
+ * *count decoded values are written. *consumed as the reference returns it.
  * On UTL_ERR_OVERFLOW, *ovf_pos = ErrOverflow.Position. */
 int utl_unpack_u32(utl_ctx *ctx, const uint8_t *src, size_t len, uint32_t *dst, int32_t *count,
                    int64_t *consumed, int64_t *ovf_pos);
@@ -108,7 +109,8 @@ int utl_get_u32(utl_ctx *ctx, int64_t pos, const uint8_t *src, size_t len, uint3
 int utl_pack_u64(utl_ctx *ctx, uint32_t flag, const uint64_t *values, size_t n, uint8_t *dst,
                  size_t dst_cap, size_t *out_len);
 
-/* uint64.go:338 UnpackUint64. dst must hold 128 values. */
+/* uint64.go:338 UnpackUint64. dst must hold 128 values or at least the
+ * block's count (src[0]); only the *count decoded values are written. */
 int utl_unpack_u64(utl_ctx *ctx, const uint8_t *src, size_t len, uint64_t *dst, int32_t *count,
                    int64_t *consumed, int64_t *ovf_pos);
 

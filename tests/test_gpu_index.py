@@ -88,3 +88,82 @@ def test_wave_index_build(ctx, n):
     got = dw.to_numpy()
     assert np.array_equal(got[0:2 * nw:2], wb[0:n:8]) and np.array_equal(got[1:2 * nw:2], wo[0:n:8])
     assert np.all(got[2 * nw:] == np.uint64(0xEEEEEEEEEEEEEEEE))  # nothing past the last entry
+
+
+def _block_corpus(orc, rng):
+    """Blocks for the header-parse pass: valid uint32 blocks of every flag
+    combination and uint64 blocks (FOR64, fit-32, two-block with its
+    block2Len), then damaged copies: reserved header bits (18, 19, 20), bit
+    widths above 32, counts above 128 and count 0. Each is paired with
+    limits: the whole block, truncations (below the header, below svbLen,
+    below the payload offset of a two-block block) and extra bytes after."""
+    blobs = []
+    for flag in range(16):
+        for n in (1, 5, 64, 128):
+            v = rng.integers(0, 1 << int(rng.integers(0, 33)), n, dtype=np.uint64).astype(np.uint32)
+            v[rng.random(n) < 0.1] = 0xFFFFFF00
+            blobs.append(orc.PackUint32(flag, v)[0])
+    for v in (np.uint64(1 << 40) + np.arange(100, dtype=np.uint64),
+              np.arange(77, dtype=np.uint64),
+              rng.integers(0, 1 << 64, 128, dtype=np.uint64, endpoint=False)):
+        for flag in (0, 1, 4):
+            blobs.append(orc.PackUint64(flag, v)[0])
+    out = []
+    for b in blobs:
+        b = bytearray(b)
+        variants = [bytes(b)]
+        for bit in (18, 19, 20):
+            x = bytearray(b)
+            x[bit // 8] |= 1 << (bit % 8)
+            variants.append(bytes(x))
+        x = bytearray(b)
+        x[1] = (x[1] & 0xE0) | int(rng.integers(9, 32))  # bw field 9..31 -> width 36..124
+        variants.append(bytes(x))
+        x = bytearray(b)
+        x[0] = int(rng.integers(129, 256))
+        variants.append(bytes(x))
+        x = bytearray(b)
+        x[0] = 0
+        variants.append(bytes(x))
+        for x in variants:
+            for lim in {len(x), 0, 3, 4, 5, 6, 9, int(rng.integers(0, len(x) + 1)), len(x) + 7}:
+                out.append((x, lim))
+    return out
+
+
+def test_block_lengths_pass_vs_oracle(ctx, orc):
+    """utl_block_lengths (the device header-parse pass) against the oracle's
+    BlockLength (block_length.go:8-65) on every (block, limit) pair: status
+    and meta word (length | count << 24, 0 on error)."""
+    from utlpfor_amd import _lib
+    from utlpfor_amd.device import DeviceArray, check
+    rng = np.random.default_rng(41)
+    corpus = _block_corpus(orc, rng)
+    buf = bytearray(16)
+    starts, limits = [], []
+    for x, lim in corpus:
+        buf += bytes(int(rng.integers(0, 4)))  # arbitrary alignment
+        starts.append(len(buf))
+        buf += x + bytes(8)
+        limits.append(starts[-1] + min(lim, len(x) + 8))
+    buf += bytes(16)
+    n = len(corpus)
+    say(f"header-parse pass: {n} (block, limit) pairs")
+    d_src = DeviceArray.from_numpy(np.frombuffer(bytes(buf), np.uint8), ctx)
+    d_start = DeviceArray.from_numpy(np.array(starts, np.uint64), ctx)
+    d_lim = DeviceArray.from_numpy(np.array(limits, np.uint64), ctx)
+    d_meta = DeviceArray(n, np.uint32, ctx)
+    d_st = DeviceArray(n, np.int32, ctx)
+    check(_lib.load().utl_block_lengths(ctx.handle, d_src.ptr, d_start.ptr, d_lim.ptr, n, d_meta.ptr, d_st.ptr),
+          "utl_block_lengths")
+    ctx.sync()
+    meta, st = d_meta.to_numpy(), d_st.to_numpy()
+    seen = set()
+    for k in range(n):
+        src = bytes(buf[starts[k]:limits[k]])
+        bl, err = orc.BlockLength(src)
+        assert st[k] == err, (k, src[:8].hex(), len(src))
+        want = ((bl & 0xFFFFFF) | (src[0] << 24)) if err == 0 else 0
+        assert meta[k] == want, (k, src[:8].hex(), len(src))
+        seen.add(err)
+    assert seen >= {0, 1, 3, 4}  # OK, InvalidBuffer, InvalidBlockLength, InvalidFlags all exercised

@@ -134,27 +134,41 @@ def PackUint64(flag: int, values, dst=None, scratch=None, ctx: Context | None = 
     return _pack(True, flag, values, dst, ctx)
 
 
-def _unpack(is64, src, ctx):
+def _unpack(is64, src, values, ctx):
+    """Decodes into the caller's array when it can hold the result, as the
+    reference does: uint32 when its capacity is at least 128 values
+    (growUnpackDst, unpack_block.go:177-182), uint64 when it is at least the
+    block's count (uint64.go:224-227); otherwise into a new array. The
+    returned values are then a view of the caller's array."""
     lib = _lib.load()
     _a, p = _buf(src)
-    dst = np.zeros(128, dtype=np.uint64 if is64 else np.uint32)
+    dt = np.uint64 if is64 else np.uint32
+    need = (src[0] if len(src) else 128) if is64 else 128
+    if (isinstance(values, np.ndarray) and values.dtype == dt and values.ndim == 1 and values.flags.c_contiguous
+            and values.flags.writeable and values.size >= max(need, 1)):
+        dst = values
+    else:
+        dst = np.zeros(128, dtype=dt)
     c, cons, ovf = C.c_int32(), C.c_int64(), C.c_int64()
     f = lib.utl_unpack_u64 if is64 else lib.utl_unpack_u32
     code = f(_h(ctx), p, len(src), dst.ctypes.data_as(_lib.u64p if is64 else _lib.u32p), C.byref(c),
              C.byref(cons), C.byref(ovf))
     if code:
         return None, 0, error_from_code(code, ovf.value)
-    return dst[: c.value].copy(), cons.value, None
+    return dst[: c.value], cons.value, None
 
 
 def UnpackUint32(src, values=None, scratch=None, ctx: Context | None = None):
-    """unpack.go:5. Returns (values, consumed, err)."""
-    return _unpack(False, src, ctx)
+    """unpack.go:5. Returns (values, consumed, err). A uint32 numpy `values`
+    of at least 128 elements receives the decoded values (the result is a
+    view of it)."""
+    return _unpack(False, src, values, ctx)
 
 
 def UnpackUint64(src, values=None, scratch=None, ctx: Context | None = None):
-    """uint64.go:338. Returns (values, consumed, err)."""
-    return _unpack(True, src, ctx)
+    """uint64.go:338. Returns (values, consumed, err). A uint64 numpy `values`
+    with room for the block's count receives the decoded values."""
+    return _unpack(True, src, values, ctx)
 
 
 def _get(is64, pos, src, ctx):
