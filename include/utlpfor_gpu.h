@@ -284,6 +284,19 @@ int utl_gather_index(utl_ctx *ctx, const uint64_t *d_block_off, const uint64_t *
 #define UTL_SLOT_BYTES64 2048u
 int utl_encode_blocks(utl_ctx *ctx, int is_u64, const void *d_values, const uint64_t *d_in_off,
                       const uint32_t *d_in_meta, size_t nblocks, uint8_t *d_slots, uint32_t *d_out_meta);
+/* Batched PackUint32 straight into the concatenated stream (no slots and no
+ * compaction): each workgroup encodes 32 consecutive jobs, learns its byte
+ * offset from the ones before it by a decoupled look-back over tiles and
+ * writes its blocks at their place in d_dst. Same job table as
+ * utl_encode_blocks; d_dst, d_out_meta and d_block_off[0..nblocks] receive
+ * what utl_encode_blocks + utl_index_build + utl_compact give, byte for
+ * byte (bytes of d_dst past the stream are not written). dst_cap must be at
+ * least nblocks * utl_max_block_length32(0). Synchronous: returns the
+ * stream's length in *out_nbytes. UTL_ERR_HIP if a tile's look-back did not
+ * complete (its waits are bounded; nothing of such a tile is written). */
+int utl_encode_u32_direct(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *d_in_off,
+                          const uint32_t *d_in_meta, size_t nblocks, uint8_t *d_dst, size_t dst_cap,
+                          uint32_t *d_out_meta, uint64_t *d_block_off, size_t *out_nbytes);
 /* Concatenates encoded slots into d_dst at d_block_off (from utl_index_build
  * over d_out_meta): byte-identical to appending the blocks with Append.
  * Writes whole aligned dwords: d_dst must hold d_block_off[nblocks] bytes

@@ -35,13 +35,14 @@ def _meta_of(bl, bc):
     return (bl.astype(np.uint32) & 0xFFFFFF) | (bc.astype(np.uint32) << 24)
 
 
-def test_delta64m_encode_bytes_and_decode(ctx, orc):
+@pytest.mark.parametrize("method", ["slot", "direct"])
+def test_delta64m_encode_bytes_and_decode(ctx, orc, method):
     from utlpfor_amd import batch, workloads
     v = workloads.delta_postings(64 << 20)
     say(f"delta postings: {v.size} values; oracle pack")
     want, bl, bc = orc.pack_stream(orc.Delta, v, nthreads=16)
-    say("GPU encode")
-    s = batch.encode(ctx, v, flag=1)
+    say(f"GPU encode ({method})")
+    s = batch.encode(ctx, v, flag=1, method=method)
     assert s.nbytes == want.size and s.nblocks == bl.size
     assert np.array_equal(s.meta.to_numpy(), _meta_of(bl, bc))
     assert np.array_equal(s.bytes_host(), want)
@@ -53,7 +54,8 @@ def test_delta64m_encode_bytes_and_decode(ctx, orc):
     s.free()
 
 
-def test_uneven_20000_streams_encode_bytes(ctx, orc):
+@pytest.mark.parametrize("method", ["slot", "direct"])
+def test_uneven_20000_streams_encode_bytes(ctx, orc, method):
     from utlpfor_amd import batch, workloads
     v, lens, flags = workloads.uneven_streams(20_000, seed=3)
     vo = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
@@ -64,8 +66,8 @@ def test_uneven_20000_streams_encode_bytes(ctx, orc):
 
     with ThreadPoolExecutor(16) as ex:
         parts = list(ex.map(pack, range(lens.size)))
-    say("GPU encode")
-    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    say(f"GPU encode ({method})")
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags, method=method)
     got = s.bytes_host()
     meta = s.meta.to_numpy()
     boff = 0

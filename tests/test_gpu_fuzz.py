@@ -265,17 +265,20 @@ def _edge_values(rng, n, is64):
 
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("is64", [False, True])
-def test_fuzz_encode_bytes(ctx, orc, seed, is64):
-    """GPU encode of many short streams with every flag combination: the
-    packed bytes equal the oracle's PackUint32/64(flag|Append) stream by
-    stream, and the meta words give its block lengths and counts."""
+@pytest.mark.parametrize("method", ["slot", "direct"])
+def test_fuzz_encode_bytes(ctx, orc, seed, is64, method):
+    """GPU encode of many short streams with every flag combination, in both
+    encode forms (uint64 has the slot form only): the packed bytes equal the
+    oracle's PackUint32/64(flag|Append) stream by stream, and the meta words
+    give its block lengths and counts."""
     from utlpfor_amd import batch
     rng = np.random.default_rng(3000 + seed)
     lens = rng.integers(1, 700, int(rng.integers(30, 80)))
     flags = rng.choice(8, lens.size).astype(np.uint8)
     parts = [_edge_values(rng, int(L), is64) for L in lens]
-    say(f"encode fuzz seed {seed} {'u64' if is64 else 'u32'}: {lens.size} streams")
-    s = batch.encode(ctx, np.concatenate(parts), stream_lengths=lens.astype(np.uint64), stream_flags=flags)
+    say(f"encode fuzz seed {seed} {'u64' if is64 else 'u32'} {method}: {lens.size} streams")
+    s = batch.encode(ctx, np.concatenate(parts), stream_lengths=lens.astype(np.uint64), stream_flags=flags,
+                     method=method)
     got, meta = s.bytes_host(), s.meta.to_numpy()
     off = b = 0
     for p, f in zip(parts, flags):

@@ -158,3 +158,44 @@ def tune_decode(ctx: Context, s: batch.DeviceStream, out: DeviceArray | None = N
         cache[key] = rec
         _save(cache)
     return dict(rec, candidates=results, cached=False)
+
+
+# ---- encode form: slots + compaction, or direct writes ----
+
+def _encode_key(ctx: Context, nblocks: int, nvalues: int) -> str:
+    name, cus = ctx.device_info()
+    return f"encode|{name}|{cus}|nb={int(np.log2(max(nblocks, 1)))}|vpb={int(np.log2(max(nvalues / max(nblocks, 1), 1)))}"
+
+
+def cached_encode_method(ctx: Context, nblocks: int, nvalues: int):
+    """The encode form cached for this shape of work, or None."""
+    rec = _load().get(_encode_key(ctx, nblocks, nvalues))
+    return rec["method"] if rec else None
+
+
+def tune_encode(ctx: Context, values: DeviceArray, stream_lengths=None, stream_flags=None, flag: int = 0,
+                reps: int = 3, use_cache: bool = True) -> dict:
+    """Times both uint32 encode forms on the caller's resident values (the
+    whole batch.encode: job table, encode, index, compaction for the slot
+    form), in turn, keeps the faster (median of reps) and caches the choice
+    for batch.encode(method="auto"). Returns {"method", "slot_ms",
+    "direct_ms"}."""
+    import time
+    times = {m: [] for m in batch.ENCODE_METHODS}
+    nblocks = 0
+    for _ in range(reps + 1):
+        for m in batch.ENCODE_METHODS:
+            ctx.sync()
+            t0 = time.perf_counter()
+            s = batch.encode(ctx, values, flag=flag, stream_lengths=stream_lengths, stream_flags=stream_flags,
+                             method=m)
+            times[m].append((time.perf_counter() - t0) * 1e3)
+            nblocks = s.nblocks
+            s.free()
+    med = {m: float(np.median(t[1:])) for m, t in times.items()}  # the first round warms up
+    rec = {"method": min(med, key=med.get), "slot_ms": med["slot"], "direct_ms": med["direct"]}
+    if use_cache:
+        cache = _load()
+        cache[_encode_key(ctx, nblocks, values.n)] = rec
+        _save(cache)
+    return rec

@@ -18,6 +18,7 @@ from . import _lib
 from .device import Context, DeviceArray, check
 
 SLOT32, SLOT64 = 1024, 2048
+MAX_BLOCK32 = 1018  # MaxBlockLength32(0), the longest uint32 block (scalar.go:336-357)
 
 
 @dataclass
@@ -54,10 +55,28 @@ def build_index(ctx: Context, meta: DeviceArray, nblocks: int):
     return boff, ooff
 
 
+ENCODE_METHODS = ("slot", "direct")
+
+
+def encode_method(ctx: Context, is64: bool, nblocks: int, nvalues: int, method: str = "auto") -> str:
+    """The encode form to use: "slot" (each block into a fixed slot, then the
+    index scan and a compaction pass) or "direct" (uint32 only: blocks
+    written straight to their place, offsets by look-back). "auto" takes the
+    autotuner's cached choice for this shape of work, else direct for
+    uint32."""
+    if is64:
+        return "slot"
+    if method in ENCODE_METHODS:
+        return method
+    from . import autotune
+    return autotune.cached_encode_method(ctx, nblocks, nvalues) or "direct"
+
+
 def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flags=None,
-           is64: bool | None = None) -> DeviceStream:
+           is64: bool | None = None, method: str = "auto") -> DeviceStream:
     """Encodes one stream (or several, back to back) on the GPU. `values` is a
-    numpy array or a DeviceArray of uint32/uint64."""
+    numpy array or a DeviceArray of uint32/uint64. `method`: see
+    encode_method; every form gives identical bytes."""
     lib = _lib.load()
     dv = values if isinstance(values, DeviceArray) else DeviceArray.from_numpy(np.ascontiguousarray(values), ctx)
     if is64 is None:
@@ -78,17 +97,30 @@ def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flag
     in_meta = DeviceArray(nblocks, np.uint32, ctx)
     check(lib.utl_make_jobs(ctx.handle, d_val_off.ptr, d_blk_off.ptr, d_flags.ptr, stream_lengths.size, in_off.ptr,
                             in_meta.ptr), "utl_make_jobs")
-    slot = SLOT64 if is64 else SLOT32
-    slots = DeviceArray(nblocks * slot, np.uint8, ctx)
     meta = DeviceArray(nblocks, np.uint32, ctx)
-    check(lib.utl_encode_blocks(ctx.handle, int(is64), dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr,
-                                meta.ptr), "utl_encode_blocks")
-    boff, ooff = build_index(ctx, meta, nblocks)
-    total = int(boff.to_numpy()[-1]) if nblocks else 0
-    data = DeviceArray(total + 64, np.uint8, ctx)
-    check(lib.utl_compact(ctx.handle, int(is64), slots.ptr, boff.ptr, nblocks, data.ptr), "utl_compact")
+    if encode_method(ctx, bool(is64), nblocks, n, method) == "direct":
+        # the stream written in place; the index scan then gives both offsets
+        data = DeviceArray(nblocks * MAX_BLOCK32 + 64, np.uint8, ctx)
+        dboff = DeviceArray(nblocks + 1, np.uint64, ctx)
+        nb_out = C.c_size_t()
+        check(lib.utl_encode_u32_direct(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, data.ptr, data.nbytes,
+                                        meta.ptr, dboff.ptr, C.byref(nb_out)), "utl_encode_u32_direct")
+        dboff.free()
+        boff, ooff = build_index(ctx, meta, nblocks)
+        total = nb_out.value
+        scratch = ()
+    else:
+        slot = SLOT64 if is64 else SLOT32
+        slots = DeviceArray(nblocks * slot, np.uint8, ctx)
+        check(lib.utl_encode_blocks(ctx.handle, int(is64), dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr,
+                                    meta.ptr), "utl_encode_blocks")
+        boff, ooff = build_index(ctx, meta, nblocks)
+        total = int(boff.to_numpy()[-1]) if nblocks else 0
+        data = DeviceArray(total + 64, np.uint8, ctx)
+        check(lib.utl_compact(ctx.handle, int(is64), slots.ptr, boff.ptr, nblocks, data.ptr), "utl_compact")
+        scratch = (slots,)
     ctx.sync()
-    for a in (slots, in_off, in_meta, d_val_off, d_blk_off, d_flags):
+    for a in (*scratch, in_off, in_meta, d_val_off, d_blk_off, d_flags):
         a.free()
     if values is not dv:
         dv.free()

@@ -641,6 +641,44 @@ int utl_encode_blocks(utl_ctx* ctx, int is_u64, const void* d_values, const uint
   return UTL_OK;
 }
 
+int utl_encode_u32_direct(utl_ctx* ctx, const uint32_t* d_values, const uint64_t* d_in_off, const uint32_t* d_in_meta,
+                          size_t nblocks, uint8_t* d_dst, size_t dst_cap, uint32_t* d_out_meta, uint64_t* d_block_off,
+                          size_t* out_nbytes) {
+  if (out_nbytes) *out_nbytes = 0;
+  if (!d_block_off || (nblocks && (!d_values || !d_in_off || !d_in_meta || !d_dst || !d_out_meta)))
+    return UTL_ERR_ARGUMENT;
+  if (nblocks && dst_cap / (size_t)utl_max_block_length32(0) < nblocks) {
+    g_last_error = "utl_encode_u32_direct: dst_cap must be at least nblocks * utl_max_block_length32(0)";
+    return UTL_ERR_ARGUMENT;
+  }
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  if (nblocks == 0) {
+    HIP_TRY(utl_launch_fill_bytes((uint8_t*)d_block_off, 0, sizeof(uint64_t), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return UTL_OK;
+  }
+  if ((err = ensure_tmp(c, utl_encode_direct_tmp_bytes(nblocks)))) return err;
+  uint64_t* lb = (uint64_t*)c->d_tmp;
+  HIP_TRY(utl_launch_encode_u32_direct(d_values, d_in_off, d_in_meta, nblocks, d_dst, dst_cap, d_out_meta, d_block_off,
+                                       nullptr, lb, c->stream));
+  // the look-back's error word and the stream's length
+  const uint64_t ntiles = (nblocks + 31) / 32;
+  HIP_TRY(hipMemcpyAsync(c->h_stage, lb + 1 + ntiles, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->h_stage + 8, d_block_off + nblocks, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  uint64_t w[2];
+  memcpy(w, c->h_stage, 16);
+  if (w[0]) {
+    g_last_error = (w[0] & 1) ? "utl_encode_u32_direct: a tile's look-back for its stream offset did not complete"
+                              : "utl_encode_u32_direct: a block would have ended past dst_cap";
+    return UTL_ERR_HIP;
+  }
+  if (out_nbytes) *out_nbytes = (size_t)w[1];
+  return UTL_OK;
+}
+
 int utl_compact(utl_ctx* ctx, int is_u64, const uint8_t* d_slots, const uint64_t* d_block_off, size_t nblocks,
                 uint8_t* d_dst) {
   if (nblocks && (!d_slots || !d_block_off || !d_dst)) return UTL_ERR_ARGUMENT;

@@ -165,4 +165,14 @@ __device__ __forceinline__ uint32_t g8_incl_scan(uint32_t x) {
   return x;
 }
 
+// Inclusive scan over the wave's eight 8-lane groups of a value that is equal
+// within each group (DPP row_shr:8, row_bcast:15, row_bcast:31). Must run
+// with all 64 lanes active.
+__device__ __forceinline__ uint32_t wave_group_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+  return x;
+}
+
 }  // namespace utl

@@ -10,6 +10,16 @@
 hipError_t utl_launch_encode_u32(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  uint32_t* work_out, hipStream_t stream);
+// Direct form of the u32 encode: blocks written straight to their place in
+// the concatenated stream dst (byte-identical to slots + compaction), with
+// their offsets in block_off[0..nblocks]. lb is scratch of
+// utl_encode_direct_tmp_bytes(nblocks) bytes; after the launch its last word
+// is nonzero if a tile's look-back failed (nothing of that tile written) or
+// a block would have ended past dst_cap (that block not written).
+size_t utl_encode_direct_tmp_bytes(uint64_t nblocks);
+hipError_t utl_launch_encode_u32_direct(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                        uint64_t nblocks, uint8_t* dst, uint64_t dst_cap, uint32_t* out_meta,
+                                        uint64_t* block_off, uint32_t* work_out, uint64_t* lb, hipStream_t stream);
 hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  hipStream_t stream);

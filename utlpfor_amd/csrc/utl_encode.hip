@@ -341,26 +341,271 @@ __device__ __forceinline__ void pack_any(int bw, const uint32_t* v0, const uint3
   }
 }
 
-// Encodes blocks [8w, 8w + 8) of a u32 job table (wavefront w) into their
-// slots, following packBlockScalar (scalar.go:126-235): FOR decision
-// (for.go:95-118), per-lane delta with zigzag of all values on any decrease
-// (delta.go:23-47), bit width by the cost model (select_bitwidth.go:41-121),
-// UTL payload, exceptions as a sorted position list or a bitmap with their
-// high bits StreamVByte-encoded (exceptions.go:138-186).
+// ---- writing a block at an arbitrary byte offset (the direct form) ----
+//
+// A block of the concatenated stream starts at any byte. Its bytes are
+// written in three parts, each byte exactly once and no byte outside the
+// block: the header fields (header, svbLen, FOR base: at most 10 bytes) as
+// byte stores; the payload as aligned dwords assembled from the lane words
+// (each thread holds the words of lanes 2j and 2j+1; the word before a
+// thread's first one comes from its neighbour by DPP), with byte stores for
+// the partial dwords at its two ends; the exception bytes from LDS the same
+// way. Partial dwords at block boundaries are shared with the neighbouring
+// blocks, which write their own bytes of them.
+
+__device__ __forceinline__ void put_bytes_g(uint8_t* __restrict__ p, uint32_t v, int n) {
+  for (int i = 0; i < n; ++i) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+// Payload of lanes 2j, 2j+1 (scalar.go:7-59) written to global byte address
+// P (any alignment): word w of lane L is payload dword 16w + L.
+template <int BW>
+__device__ __forceinline__ void pack_bw_at(const uint32_t* v0, const uint32_t* v1, uint8_t* __restrict__ g,
+                                           uint64_t P, int j) {
+  if constexpr (BW > 0) {
+    constexpr int NW = BW / 4;
+    const uint32_t s = (uint32_t)(P & 3);
+    uint32_t* a32 = reinterpret_cast<uint32_t*>(g + (P - s));  // aligned dword holding payload byte 0
+    uint32_t prev_c = 0;  // this thread's odd word of the previous w
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      uint32_t a = 0, c = 0;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int sft = r * BW - 32 * w;
+        if (sft > -BW && sft < 32) {
+          a |= sft >= 0 ? (v0[r] << sft) : (v0[r] >> (-sft));
+          c |= sft >= 0 ? (v1[r] << sft) : (v1[r] >> (-sft));
+        }
+      }
+      const int qe = 16 * w + 2 * j;
+      if (s == 0) {
+        a32[qe] = a;
+        a32[qe + 1] = c;
+      } else {
+        // payload dword q - 1 for q = qe: lane 2j - 1's word w (the thread
+        // before, DPP row_shr:1) or, for j = 0, lane 15's word w - 1 (thread
+        // 7 of the group, DPP row_shl:7 of its previous odd word)
+        const uint32_t from_left = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)c, 0x111, 0xF, 0xF, true);
+        const uint32_t from_7 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)prev_c, 0x107, 0xF, 0xF, true);
+        const uint32_t before = j ? from_left : from_7;
+        a32[qe + 1] = __builtin_amdgcn_alignbyte(c, a, 4 - s);
+        if (qe > 0)
+          a32[qe] = __builtin_amdgcn_alignbyte(a, before, 4 - s);
+        else
+          put_bytes_g(g + P, a, 4 - (int)s);  // payload bytes 0 .. 3 - s
+        if (w == NW - 1 && j == 7) put_bytes_g(g + P + 16 * BW - s, c >> (8 * (4 - s)), (int)s);  // the last s
+      }
+      prev_c = c;
+    }
+  }
+}
+
+__device__ __forceinline__ void pack_any_at(int bw, const uint32_t* v0, const uint32_t* v1, uint8_t* __restrict__ g,
+                                            uint64_t P, int j) {
+  switch (bw) {
+    case 4: pack_bw_at<4>(v0, v1, g, P, j); break;
+    case 8: pack_bw_at<8>(v0, v1, g, P, j); break;
+    case 12: pack_bw_at<12>(v0, v1, g, P, j); break;
+    case 16: pack_bw_at<16>(v0, v1, g, P, j); break;
+    case 20: pack_bw_at<20>(v0, v1, g, P, j); break;
+    case 24: pack_bw_at<24>(v0, v1, g, P, j); break;
+    case 28: pack_bw_at<28>(v0, v1, g, P, j); break;
+    case 32: pack_bw_at<32>(v0, v1, g, P, j); break;
+    default: break;
+  }
+}
+
+// n bytes of a 4-byte aligned LDS buffer written to global byte address T
+// (any alignment) by the 8 threads of a group.
+__device__ __forceinline__ void copy_lds_at(const uint8_t* tl, int n, uint8_t* __restrict__ g, uint64_t T, int j) {
+  const uint32_t* t32 = reinterpret_cast<const uint32_t*>(tl);
+  const int s = (int)(T & 3);
+  uint32_t* a32 = reinterpret_cast<uint32_t*>(g + (T - s));
+  const int nd = (s + n + 3) >> 2;  // aligned dwords touched
+  for (int k = j; k < nd; k += 8) {
+    const int b = 4 * k - s;  // the first tail byte of dword k (may be < 0)
+    if (b >= 0 && b + 4 <= n) {
+      a32[k] = s == 0 ? t32[k] : __builtin_amdgcn_alignbyte(t32[k], t32[k - 1], 4 - s);
+    } else {
+      // a partial dword: only the tail's own bytes
+      const int lo = b < 0 ? 0 : b, hi = min(b + 4, n);
+      for (int i = lo; i < hi; ++i) g[T + i] = tl[i];
+    }
+  }
+}
+
+// ---- decoupled look-back over workgroup tiles (the direct form) ----
+//
+// lb[0] is the tile ticket counter, lb[1 + t] the status word of tile t
+// (value << 2 | flag: 1 the tile's own byte total, 2 the inclusive prefix,
+// 3 failed), lb[1 + ntiles] the error word; the host zero-fills all of them
+// before the launch. Workgroups take tiles in the order they start (the
+// ticket), so every tile a workgroup waits for belongs to a workgroup that is
+// already running. Waits are bounded: a tile that does not see its
+// predecessors' totals within kSpinLimit polls, or sees a failed one, marks
+// itself failed, writes nothing and sets the error word, so the kernel always
+// ends.
+constexpr uint32_t kLbAgg = 1, kLbPrefix = 2, kLbFailed = 3;
+constexpr uint32_t kSpinLimit = 1u << 20;
+
+// A status word carries all its information (flag and value in one 64-bit
+// word, nothing else is read because of it), so its loads and stores are
+// relaxed device-scope atomics: an acquire would also invalidate the L2 of
+// the polling workgroup's XCD on every poll.
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive byte offset of tile t whose own total is agg, found by the
+// whole workgroup (every thread calls both halves). lookback_begin publishes
+// the tile's total and issues the loads of the first window; the caller
+// does other work (the exception bytes of its blocks) while they are in
+// flight; lookback_finish resolves the window. Its 256 threads read the
+// status words of the 512 tiles before the window's end at once (two each),
+// so one round trip covers 512 tiles (the nearest prefix is almost always in
+// the first window; four words each would take the kernel above 128 VGPRs). In a window, the nearest tile
+// with a prefix ends the walk and the tiles after it add their totals. A
+// window with a tile not yet published is read again (measured: about three
+// times per tile when the tile's own wait began right after its totals, as
+// older tiles still finish theirs); a window of totals only moves the walk
+// back by its width. sh is LDS of 3 * 4 words. ok is cleared on failure.
+constexpr int kLbThreads = 64 * 4;
+constexpr int kLbPer = 2;                      // status words per thread
+constexpr int kLbWindow = kLbThreads * kLbPer;  // tiles per round trip
+
+struct LbWindow {
+  uint64_t v[kLbPer];
+  int64_t hi;  // the window is tiles [hi - kLbWindow, hi); entry i is tile hi - 1 - i
+};
+
+__device__ __forceinline__ void lb_read(const uint64_t* __restrict__ st, LbWindow& w) {
+#pragma unroll
+  for (int q = 0; q < kLbPer; ++q) {
+    const int64_t k = w.hi - 1 - (int64_t)(threadIdx.x + kLbThreads * q);
+    w.v[q] = k >= 0 ? lb_load(&st[k]) : (uint64_t)kLbPrefix;  // before tile 0: a prefix of 0
+  }
+}
+
+__device__ __forceinline__ void lookback_begin(uint64_t* __restrict__ st, uint64_t t, uint64_t agg, LbWindow& w) {
+  if (threadIdx.x == 0) lb_store(&st[t], (agg << 2) | (t == 0 ? kLbPrefix : kLbAgg));
+  w.hi = (int64_t)t;
+  if (t > 0) lb_read(st, w);
+}
+
+__device__ uint64_t lookback_finish(uint64_t* __restrict__ st, uint64_t t, uint64_t agg, LbWindow& w, bool& ok,
+                                    uint64_t* sh) {
+  const int tid = threadIdx.x, wv = tid >> 6;
+  uint64_t excl = 0;
+  bool good = true;
+  uint32_t spins = 0;
+  while (w.hi > 0) {
+    // the nearest prefix: the lowest entry index holding one
+    uint32_t mine = kLbWindow;
+#pragma unroll
+    for (int q = kLbPer - 1; q >= 0; --q)
+      if ((w.v[q] & 3) == kLbPrefix) mine = tid + kLbThreads * q;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mine = min(mine, (uint32_t)__shfl_xor((int)mine, d, 64));
+    if ((tid & 63) == 0) sh[wv] = mine;
+    __syncthreads();
+    uint64_t ip = kLbWindow;
+#pragma unroll
+    for (int x = 0; x < kLbThreads / 64; ++x) ip = min(ip, sh[x]);
+    // the entries after the prefix (indices below ip) must all be totals
+    bool missing = false, failed = false;
+    uint64_t part = 0;
+#pragma unroll
+    for (int q = 0; q < kLbPer; ++q) {
+      const uint64_t i = (uint64_t)(tid + kLbThreads * q);
+      const uint32_t f = (uint32_t)(w.v[q] & 3);
+      if (i < ip) {
+        missing |= f != kLbAgg;
+        failed |= f == kLbFailed;
+      }
+      if (i <= ip) part += w.v[q] >> 2;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+    const uint64_t mb = __ballot(missing), fb = __ballot(failed);  // (all lanes vote)
+    if ((tid & 63) == 0) {
+      sh[4 + wv] = part;
+      sh[8 + wv] = (mb ? 1u : 0u) | (fb ? 2u : 0u);
+    }
+    __syncthreads();
+    uint64_t sum = 0, bad = 0;
+#pragma unroll
+    for (int x = 0; x < kLbThreads / 64; ++x) {
+      sum += sh[4 + x];
+      bad |= sh[8 + x];
+    }
+    __syncthreads();  // sh is written again in the next round
+    if (bad & 2) {
+      good = false;
+      break;
+    }
+    if (bad & 1) {  // a tile of the window has not published yet: read it again
+      if (++spins > kSpinLimit) {
+        good = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      lb_read(st, w);
+      continue;
+    }
+    spins = 0;
+    excl += sum;
+    if (ip < (uint64_t)kLbWindow) break;  // reached a prefix
+    w.hi -= kLbWindow;                    // totals only: the next window back
+    if (w.hi > 0) lb_read(st, w);
+  }
+  ok = good;
+  if (tid == 0 && t != 0) lb_store(&st[t], good ? (((excl + agg) << 2) | kLbPrefix) : kLbFailed);
+  return excl;
+}
+
+// Encodes the blocks of a u32 job table, eight per wavefront, following
+// packBlockScalar (scalar.go:126-235): FOR decision (for.go:95-118), per-lane
+// delta with zigzag of all values on any decrease (delta.go:23-47), bit
+// width by the cost model (select_bitwidth.go:41-121), UTL payload,
+// exceptions as a sorted position list or a bitmap with their high bits
+// StreamVByte-encoded (exceptions.go:138-186).
+// Slot form (!DIRECT): block b goes to slot b (slot_bytes apart) of dst.
+// Direct form: the workgroup's 32 blocks go straight to their place in the
+// concatenated stream in dst, found by the look-back over tiles; block_off[b]
+// receives each block's offset (block_off[nblocks] the stream's length).
+template <bool DIRECT>
 __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t* __restrict__ values,
                                                                   const uint64_t* __restrict__ in_off,
                                                                   const uint32_t* __restrict__ in_meta,
-                                                                  uint64_t nblocks, uint8_t* __restrict__ slots,
+                                                                  uint64_t nblocks, uint8_t* __restrict__ dst,
                                                                   uint32_t slot_bytes, uint32_t* __restrict__ out_meta,
-                                                                  uint32_t* __restrict__ work_out) {
+                                                                  uint32_t* __restrict__ work_out,
+                                                                  uint64_t* __restrict__ lb,
+                                                                  uint64_t* __restrict__ block_off,
+                                                                  uint64_t dst_cap) {
   __shared__ EncGrpLds lds[kEncGrpWaves];
+  __shared__ uint64_t wg[1 + kEncGrpWaves];  // tile ticket, wave totals
+  __shared__ uint64_t lbsh[12];               // the look-back's reductions
+  static_assert(64 * kEncGrpWaves == kLbThreads, "the look-back runs on the whole workgroup");
   const uint32_t wave = uniform(threadIdx.x >> 6);
-  const uint64_t b0 = ((uint64_t)blockIdx.x * kEncGrpWaves + wave) * kWaveBlocksEnc;
-  if (b0 >= nblocks) return;
+  uint64_t tile = blockIdx.x;
+  if (DIRECT) {
+    if (threadIdx.x == 0) wg[0] = atomicAdd((unsigned long long*)lb, 1ull);
+    __syncthreads();
+    tile = wg[0];
+  }
+  const uint64_t b0 = (tile * kEncGrpWaves + wave) * kWaveBlocksEnc;
+  if (!DIRECT && b0 >= nblocks) return;  // (the direct form's waves all reach its barriers)
   EncGrpLds* L = &lds[wave];
   const int blk = lane_id() >> 3, j = lane_id() & 7;
   const bool mine = b0 + blk < nblocks;
-  const uint64_t b = mine ? b0 + blk : b0;
+  // (a lane without a block reads block b0's job, or the last block's when
+  // the whole wave is past the end: every read stays inside the tables)
+  const uint64_t b = mine ? b0 + blk : min(b0, nblocks - 1);
   const uint32_t meta = in_meta[b];
   const uint32_t flag = (meta >> 8) & 0xFF;
   const int n_req = (int)(meta & 0xFF);
@@ -459,12 +704,37 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
   const int fbb = for_base_bytes(fw);
   const int poff = kHdr + (has_exc ? kSvbLen : 0) + fbb;
   const int sh = (-poff) & 3;
-  uint8_t* slot = slots + b * slot_bytes;
+  uint8_t* slot = dst + (DIRECT ? 0 : b * slot_bytes);
   uint32_t* pay = reinterpret_cast<uint32_t*>(slot + sh + poff);  // 4-byte aligned
   const uint32_t m = low_mask(bw);
 
-  // ---- payload ----
-  if (ok) {
+  // ---- direct form: the tile's byte total, published before the exception
+  // bytes are assembled (their StreamVByte data bytes are a sum of code
+  // lengths), and the look-back's first window read while that work runs ----
+  LbWindow lbw;
+  uint32_t blen = 0, incl = 0;
+  uint64_t agg = 0;
+  if (DIRECT) {
+    uint32_t eb = 0;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      eb += (ok0[r] && x0[r] > m) ? (uint32_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+      eb += (ok1[r] && x1[r] > m) ? (uint32_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+    }
+    const int data_bytes = (int)g8_add(eb);
+    const int len_early =
+        poff + 16 * bw + (has_exc ? exc_index_size(exc) + svb_ctrl_count(exc) + data_bytes : 0);
+    blen = ok ? (uint32_t)len_early : 0u;
+    incl = wave_group_scan(blen);
+    if (lane_id() == 63) wg[1 + wave] = incl;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kEncGrpWaves; ++w) agg += wg[1 + w];
+    lookback_begin(lb + 1, tile, agg, lbw);
+  }
+
+  // ---- payload (the slot form; the direct form packs once its offset is known) ----
+  if (!DIRECT && ok) {
     uint32_t p0[kRows], p1[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
@@ -568,7 +838,7 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
     svb_len = ctrl + (int)total;
     lds_fence();
     // copy the tail (aligned in the slot right after the payload)
-    if (ok) {
+    if (!DIRECT && ok) {
       uint32_t* dst = pay + 4 * bw;
       const uint32_t* src = reinterpret_cast<const uint32_t*>(tl);
       const int ndw = (isz + svb_len + 3) >> 2;
@@ -576,6 +846,44 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
     }
   }
   const int len = poff + 16 * bw + (has_exc ? isz + svb_len : 0);
+  if (DIRECT) {
+    // ---- offsets: the wave's blocks in order, the workgroup's waves in
+    // order, the tile by look-back ----
+    bool good;
+    const uint64_t lbase = lookback_finish(lb + 1, tile, agg, lbw, good, lbsh);
+    if (!good && threadIdx.x == 0) atomicOr((unsigned long long*)(lb + 1 + (nblocks + 31) / 32), 1ull);
+    const uint64_t tbase = good ? lbase : ~0ull;
+    if (tbase == ~0ull || !mine) return;
+    uint64_t o = tbase + (incl - blen);
+    for (uint32_t w = 0; w < wave; ++w) o += wg[1 + w];
+    // a block that would end past dst_cap writes nothing (the caller sizes
+    // dst for the longest blocks, so only a wrong offset could get here)
+    const bool fits = o <= dst_cap && blen <= dst_cap - o;
+    if (!fits && j == 0) atomicOr((unsigned long long*)(lb + 1 + (nblocks + 31) / 32), 2ull);
+    if (ok && fits) {
+      // header fields (bytes [0, poff)): header, svbLen, FOR base
+      const uint32_t hw = encode_header(n, bw, exc, hflags);
+      const uint32_t w1 = has_exc ? ((uint32_t)svb_len | (base << 16)) : base;
+      const uint32_t w2 = has_exc ? (fbb > 2 ? base >> 16 : 0u) : 0u;
+      const uint32_t hsel = j < 4 ? hw : w1;
+      if (j < poff) dst[o + j] = (uint8_t)(hsel >> (8 * (j & 3)));
+      if (j + 8 < poff) dst[o + j + 8] = (uint8_t)(w2 >> (8 * j));
+      uint32_t p0[kRows], p1[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        p0[r] = x0[r] & m;
+        p1[r] = x1[r] & m;
+      }
+      pack_any_at(bw, p0, p1, dst, o + poff, j);
+      if (has_exc) copy_lds_at(L->tail[blk], isz + svb_len, dst, o + poff + 16 * bw, j);
+    }
+    if (j == 0) {
+      out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
+      block_off[b] = o;
+      if (b + 1 == nblocks) block_off[nblocks] = o + blen;
+    }
+    return;
+  }
   if (j == 0 && mine) {
     if (ok) {
       // bytes [0, poff): header, svbLen, FOR base, written as the slot's
@@ -649,72 +957,74 @@ __global__ __launch_bounds__(64 * kEncWaves) void k_encode_u64(const uint64_t* _
   if (t == 0) out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
 }
 
-// Little-endian dword of slot bytes [i, i + 4) (slots are 4-byte aligned; the
-// second dword is read only when the bytes straddle it, so a read ending at
-// the slot's last byte stays inside the slot).
-// Little-endian dword at byte i of a slot (i <= slot bytes - 4). Both loads
-// are unconditional (an aligned i reads its own dword twice), so a caller's
-// loads can all be in flight together.
-__device__ __forceinline__ uint32_t slot_u32(const uint8_t* __restrict__ s, uint32_t i) {
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(s) + (i >> 2);
-  const uint32_t lo = w[0], hi = w[(i & 3) ? 1 : 0];
-  return __builtin_amdgcn_alignbyte(hi, lo, i & 3);
-}
-
-// Where a block starts in its slot: after the zero bytes the encoders put
-// before it (0..3); a block's first byte, its count, is never 0.
-__device__ __forceinline__ uint32_t slot_shift(const uint8_t* __restrict__ s) {
-  const uint32_t w = *reinterpret_cast<const uint32_t*>(s);
-  return w ? min((uint32_t)__builtin_ctz(w) >> 3, 3u) : 3u;
-}
-
 // Concatenates the slots into dst (block b's bytes at dst + boff[b]) with
 // aligned dword stores: the output dword d is written by the block holding
 // byte 4d; blocks are at least 4 bytes, so its bytes past that block come
-// from the next block's slot. Eight lanes per block, eight blocks per
-// wavefront; each lane moves four dwords per round with all their slot
-// loads in flight before the first store. dst must have room for the dword
-// holding the last byte.
+// from the next block's first bytes. Eight lanes per block, eight blocks per
+// wavefront. The index entries and the first 16 bytes of the block's slot
+// and of the next one (where the slot shifts and the next block's first
+// bytes are) are loaded together, so a block costs two round trips to
+// memory: those, then every slot dword it needs (up to 64 per round, eight
+// per lane, each as two dword loads aligned in registers). (The previous
+// form chained the index, the slot shift, the next block's shift and first
+// bytes and then 32 dwords per round: 109 us for the 64 Mi delta stream.)
+// dst must have room for the dword holding the last byte.
+constexpr int kCompactPer = 8;  // output dwords per lane per round
+
 __global__ __launch_bounds__(256) void k_compact(const uint8_t* __restrict__ slots, uint32_t slot_bytes,
                                                  const uint64_t* __restrict__ boff, uint64_t nblocks,
                                                  uint8_t* __restrict__ dst) {
   const uint64_t b = ((uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6)) * 8 + (lane_id() >> 3);
   if (b >= nblocks) return;
   const int l8 = lane_id() & 7;
-  const uint64_t o = boff[b], o1 = boff[b + 1];
   const uint8_t* s = slots + b * slot_bytes;
   const bool has_next = b + 1 < nblocks;
   const uint8_t* sn = slots + (has_next ? b + 1 : b) * slot_bytes;
-  const uint32_t sh = slot_shift(s);
-  // the first bytes of the next block, for the dword that straddles o1
-  const uint32_t next_w = has_next ? slot_u32(sn, slot_shift(sn)) : 0u;
+  const uint4 head = *reinterpret_cast<const uint4*>(s);
+  const uint4 nhead = *reinterpret_cast<const uint4*>(sn);
+  const uint64_t o = boff[b], o1 = boff[b + 1];
+  // where the blocks start in their slots: after the encoders' zero bytes
+  // (0..3); a block's first byte, its count, is never 0
+  const uint32_t sh = head.x ? min((uint32_t)__builtin_ctz(head.x) >> 3, 3u) : 3u;
+  const uint32_t nsh = nhead.x ? min((uint32_t)__builtin_ctz(nhead.x) >> 3, 3u) : 3u;
+  const uint32_t next_w = has_next ? __builtin_amdgcn_alignbyte(nhead.y, nhead.x, nsh) : 0u;
   uint32_t* d32 = reinterpret_cast<uint32_t*>(dst);
+  const uint32_t* s32 = reinterpret_cast<const uint32_t*>(s);
   const uint64_t d0 = (o + 3) >> 2, d1 = (o1 + 3) >> 2;  // dwords whose first byte is in this block
   const int ndw = (int)(d1 - d0);
+  // output dword k is slot bytes [in0 + 4k, in0 + 4k + 4): slot dwords
+  // (in0 >> 2) + k and the one after, shifted by in0 & 3
+  const uint32_t in0 = (uint32_t)(4 * d0 - o + sh);
+  const uint32_t q0 = in0 >> 2, r = in0 & 3;
   // sh + (p - o) <= 3 + MaxBlockLength - 1 for every block the encoders
   // produce; the clamp keeps reads inside the slot for an index built over
   // rejected blocks (meta 0xFFFFFFFF), whose bytes are meaningless anyway
-  const uint64_t in0 = 4 * d0 - o + sh;
-  for (int k0 = 0; k0 < ndw; k0 += 32) {
-    uint32_t w[4];
+  const uint32_t qmax = slot_bytes / 4 - 1;
+  for (int k0 = 0; k0 < ndw; k0 += 8 * kCompactPer) {
+    uint32_t lo[kCompactPer], hi[kCompactPer];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + 8 * u + l8;
-      const uint32_t in = (uint32_t)min(in0 + 4 * (uint64_t)k, (uint64_t)(slot_bytes - 4));
-      w[u] = slot_u32(s, in);  // past the block: a clamped in-slot read, not stored
+    for (int u = 0; u < kCompactPer; ++u) {
+      const uint32_t q = min(q0 + (uint32_t)(k0 + 8 * u + l8), qmax);
+      lo[u] = s32[q];
+      hi[u] = s32[min(q + 1, qmax)];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kCompactPer; ++u) {
       const int k = k0 + 8 * u + l8;
       if (k < ndw) {
         const uint64_t p = 4 * (d0 + (uint64_t)k);  // first byte of the dword, in [o, o1)
         const uint64_t mine = o1 - p;               // bytes of the dword in this block (>= 1)
-        uint32_t v = w[u];
+        uint32_t v = __builtin_amdgcn_alignbyte(hi[u], lo[u], r);
         if (mine < 4) v = (v & ((1u << (8 * mine)) - 1u)) | (next_w << (8 * mine));
         d32[d0 + k] = v;
       }
     }
   }
+}
+
+__global__ void k_fill_lb(uint64_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // Job table for streams cut into 128-value blocks: stream s covers values
@@ -755,8 +1065,26 @@ hipError_t utl_launch_encode_u32(const uint32_t* values, const uint64_t* in_off,
   if (nblocks == 0) return hipSuccess;
   const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
   const uint64_t grid = (nblocks + per_wg - 1) / per_wg;
-  k_encode_u32<<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
-                                                                               slot_bytes, out_meta, work_out);
+  k_encode_u32<false><<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, slots, slot_bytes, out_meta, work_out, nullptr, nullptr, 0);
+  return hipGetLastError();
+}
+
+size_t utl_encode_direct_tmp_bytes(uint64_t nblocks) {
+  const uint64_t ntiles = (nblocks + 31) / 32;
+  return (size_t)((ntiles + 2) * sizeof(uint64_t));
+}
+
+hipError_t utl_launch_encode_u32_direct(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                        uint64_t nblocks, uint8_t* dst, uint64_t dst_cap, uint32_t* out_meta,
+                                        uint64_t* block_off, uint32_t* work_out, uint64_t* lb, hipStream_t stream) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  const uint64_t ntiles = (nblocks + per_wg - 1) / per_wg;
+  const uint64_t nlb = ntiles + 2;
+  k_fill_lb<<<dim3((unsigned)((nlb + 255) / 256)), dim3(256), 0, stream>>>(lb, nlb);
+  k_encode_u32<true><<<dim3((unsigned)ntiles), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, dst, 0, out_meta, work_out, lb, block_off, dst_cap);
   return hipGetLastError();
 }
 
