@@ -1,6 +1,8 @@
 """Host logic of the serving loop (no GPU): admission in arrival order within
-the heap, continuous batching of late arrivals into the next tick, latency
-accounting, and the request generator's mix and ranges."""
+the heap, admission by the device's free memory (the heap grows while it
+lasts), continuous batching of late arrivals into the next tick, latency
+accounting, deliveries completed before serve() returns or raises, and the
+request generator's mix and ranges."""
 import numpy as np
 
 from utlpfor_amd import serve
@@ -25,6 +27,9 @@ def _engine(cap, clock, tick_s):
     eng.store = _FakeStore()
     eng.heap = type("H", (), {"cap": cap})()
     eng.batches = []
+    eng.growths = 0
+    eng.admission_limit = lambda half: cap  # a fixed heap
+    eng.fit_half = lambda half, need: None
 
     def launch(reqs, half):  # one tick's kernels: no deliveries
         eng.batches.append([id(r) for r in reqs])
@@ -121,3 +126,92 @@ def test_make_requests_mix_and_ranges():
             assert r.positions.max() < store.streams[r.stream].nvalues
         else:
             assert 1 <= r.values.size <= 10000
+
+
+class _Half:
+    def __init__(self, cap):
+        self.cap = cap
+
+    def grow(self, n):
+        self.cap = max(self.cap, n)
+
+
+def _growing_engine(cap, free, reserve, clock, tick_s):
+    """An engine whose admission reads a fake free-memory figure."""
+    eng = serve.Engine.__new__(serve.Engine)
+    eng.store = _FakeStore()
+    eng.heaps = [_Half(cap), _Half(cap)]
+    eng.heap = eng.heaps[0]
+    eng.grow, eng.reserve, eng.growths = True, reserve, 0
+    eng.batches, eng.halves = [], []
+    eng.free = free
+
+    def launch(reqs, half):
+        assert sum(eng.cost(r) for r in reqs) <= eng.heaps[half].cap  # fit_half ran first
+        eng.batches.append(len(reqs))
+        eng.halves.append(half)
+        clock.t += tick_s
+        return []
+
+    eng._launch = launch
+    return eng
+
+
+def test_admission_by_live_free_memory(monkeypatch):
+    """Admission follows the device's free memory at each tick: with room
+    the heap half grows and one tick takes everything that arrived; when the
+    free memory falls to the reserve, ticks are cut to the heap's size."""
+    clock = _Clock()
+    monkeypatch.setattr(serve.time, "sleep", lambda s: setattr(clock, "t", clock.t + s))
+    reqs = [serve.Request(serve.DECOMPRESS, 0.0, stream=0, first=i, nblocks=10) for i in range(8)]
+    one = serve.Engine.cost(_engine(1, clock, 0), reqs[0])
+    state = {"free": 100 * one}
+    monkeypatch.setattr(serve, "mem_info", lambda: (state["free"], 1 << 40))
+    eng = _growing_engine(2 * one, 100 * one, 0, clock, 0.01)
+    eng.serve(reqs, deliver=False, clock=clock)
+    assert eng.batches == [8] and eng.growths == 1 and eng.heaps[0].cap >= 8 * one
+    # no free memory beyond the reserve: the heap's own size bounds a tick
+    state["free"] = 0
+    eng = _growing_engine(2 * one, 0, 0, clock, 0.01)
+    for r in reqs:
+        r.done = float("nan")
+    eng.serve(reqs, deliver=False, clock=clock)
+    assert eng.batches == [2, 2, 2, 2] and eng.growths == 0
+
+
+def test_serve_finishes_deliveries_when_a_tick_fails(monkeypatch):
+    """A launch that raises on the second tick: serve() re-raises only after
+    the first tick's delivery (running on the worker thread) has completed,
+    so no later tick can reuse that tick's heap half while it is copied."""
+    import threading
+    clock = _Clock()
+    monkeypatch.setattr(serve.time, "sleep", lambda s: setattr(clock, "t", clock.t + s))
+    reqs = [serve.Request(serve.DECOMPRESS, 0.0, stream=0, first=i, nblocks=10) for i in range(4)]
+    cost = serve.Engine.cost(_engine(1, clock, 0), reqs[0])
+    eng = _engine(cost * 2, clock, 0.01)
+    started, release, delivered = threading.Event(), threading.Event(), []
+
+    def slow_delivery():
+        started.set()
+        release.wait(5)
+        delivered.append(True)
+
+    calls = []
+
+    def launch(b, half):
+        calls.append(half)
+        if len(calls) == 2:
+            assert started.wait(5)
+            threading.Timer(0.2, release.set).start()  # the delivery is still copying
+            raise MemoryError("serving heap exhausted")
+        return [slow_delivery]
+
+    eng._launch = launch
+    try:
+        eng.serve(reqs, clock=clock)
+    except MemoryError:
+        pass
+    else:
+        raise AssertionError("the launch error was swallowed")
+    assert delivered == [True]  # the first tick's delivery finished before serve() raised
+    assert reqs[0].tick == 0 and reqs[1].tick == 0

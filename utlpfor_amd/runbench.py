@@ -173,7 +173,7 @@ def serving(ctx: Context, store_bytes: int = 16 << 30, rate: float = 2000.0, nre
            "store_bytes": store.nbytes, "streams": len(store.streams), "offered_rate_per_s": rate}
     for deliver in (True, False):
         reqs = serve.make_requests(store, pool, n=nreq, rate=rate, seed=seed + 20)
-        eng = serve.Engine(ctx, store, heap_fraction=0.25)
+        eng = serve.Engine(ctx, store)
         eng.run_batch(reqs[:8], deliver)  # warm-up tick
         stats = eng.serve(reqs, deliver=deliver)
         stats["heap_bytes"] = sum(h.cap for h in eng.heaps)
@@ -185,6 +185,47 @@ def serving(ctx: Context, store_bytes: int = 16 << 30, rate: float = 2000.0, nre
             progress(f"served ({'to host' if deliver else 'in HBM'}): {stats}")
     store.free()
     return out
+
+
+def serving_sweep(ctx: Context, store_bytes: int = 16 << 30, nreq: int = 3000, seed: int = 5,
+                  rates=(2_000, 5_000, 10_000, 20_000, 50_000, 100_000, 200_000), deliver_rates=(2_000, 20_000),
+                  progress=None):
+    """Offered-load sweep of the serving loop over one resident store (the
+    workload of serving()): for each offered rate, the same kind of request
+    stream, served with results left in HBM (the GPU's capacity) and, at a
+    few rates, delivered to host memory (the PCIe link's). Yields one record
+    per point: p50/p99 latency, mean and largest batch, ticks, delivered
+    GB/s, and the sampled bit-exact check."""
+    from . import serve
+    say = progress or (lambda m: None)
+    sources = [workloads.plain_1m(8 << 20, seed=seed), workloads.delta_postings(32 << 20, seed=seed + 1),
+               workloads.timestamps64(16 << 20, seed=seed + 2), workloads.uniform64(4 << 20, seed=seed + 3)]
+    store = serve.Store(ctx, sources, [0, 1, 0, 0], target_bytes=store_bytes)
+    say(f"store: {len(store.streams)} streams, {store.nblocks} blocks, {store.nbytes / 1e9:.2f} GB")
+    pool = [(workloads.plain_1m(4 << 20, seed=seed + 10), 0), (workloads.delta_postings(8 << 20, seed=seed + 11), 1),
+            (workloads.timestamps64(4 << 20, seed=seed + 12), 0), (workloads.uniform64(1 << 20, seed=seed + 13), 0)]
+    eng = serve.Engine(ctx, store)
+    try:
+        eng.run_batch(serve.make_requests(store, pool, n=8, rate=1e6, seed=seed), True)  # warm-up tick
+        for deliver in (False, True):
+            for rate in (rates if not deliver else deliver_rates):
+                reqs = serve.make_requests(store, pool, n=nreq, rate=float(rate), seed=seed + 20)
+                stats = eng.serve(reqs, deliver=deliver)
+                rec = {"workload": "serving sweep: Poisson arrivals, 60% decompress / 25% compress / 15% Get(1-1024), "
+                                   "sizes logU[128, 4Mi] values, mixed u32/u64",
+                       "store_bytes": store.nbytes, "offered_rate_per_s": rate,
+                       "results": "to_host" if deliver else "in_hbm",
+                       "offered_GBps": round(sum(serve.request_bytes(r, store) for r in reqs) /
+                                             max(reqs[-1].arrival, 1e-9) / 1e9, 3)}
+                rec.update(stats)
+                if deliver:
+                    rec["bit_exact_sampled"] = _check_served(ctx, store, reqs[::50])
+                say(f"rate {rate}/s {'to host' if deliver else 'in HBM'}: p50 {stats['p50_ms']} ms, p99 "
+                    f"{stats['p99_ms']} ms, mean batch {stats['mean_batch']:.1f}, {stats['GBps']} GB/s")
+                yield rec
+    finally:
+        eng.close()
+        store.free()
 
 
 def _roundtrip(ctx: Context, packed: np.ndarray, values: np.ndarray) -> bool:

@@ -11,9 +11,10 @@ each request with a loop of UnpackUint32 / PackUint32 / GetUint32 calls
 * Requests: decompress a block range of a stream, compress a value array,
   or Get up to 1024 positions of a stream.
 * Engine: each tick admits the requests that have arrived, in arrival
-  order, while their device memory fits the engine's heap (a fixed share of
-  free HBM taken at start-up; admission by free HBM), then runs the whole
-  batch with one launch per kind and type: a gathered index
+  order, while their device memory fits what the device has free: the
+  tick's heap half, which grows (up to the device's free memory at that
+  moment, less a reserve) when the admitted requests need more. Then it runs
+  the whole batch with one launch per kind and type: a gathered index
   (utl_gather_index) plus one decode for all decompress requests, one
   batched Get, one batched encode. Requests that arrive meanwhile join the
   next tick. A request's latency is from its arrival to the end of the tick
@@ -119,9 +120,18 @@ class _Heap:
     """Bump allocator over one device allocation, reset every tick."""
 
     def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
         self.mem = DeviceArray(nbytes, np.uint8, ctx)
         self.cap = nbytes
         self.top = 0
+
+    def grow(self, nbytes: int) -> None:
+        """Replaces the allocation by one of nbytes (its contents are not
+        kept: only between ticks)."""
+        if nbytes > self.cap:
+            self.mem.free()
+            self.mem = DeviceArray(nbytes, np.uint8, self.ctx)
+            self.cap = nbytes
 
     def reset(self) -> None:
         self.top = 0
@@ -221,15 +231,22 @@ class Engine:
     use in turn: a half is reused only once its tick's results are
     delivered."""
 
-    def __init__(self, ctx: Context, store: Store, heap_bytes: int | None = None, heap_fraction: float = 0.5):
+    def __init__(self, ctx: Context, store: Store, heap_bytes: int | None = None, heap_fraction: float = 0.05,
+                 reserve_fraction: float = 0.05, grow: bool = True):
+        """heap_bytes (or heap_fraction of the free device memory) is the
+        heap's starting size; with grow, a tick's half grows while the
+        device keeps reserve_fraction of its memory free."""
+        free, total = mem_info()
         if heap_bytes is None:
-            free, _ = mem_info()
             heap_bytes = int(free * heap_fraction)
         self.ctx, self.store = ctx, store
         self.lib = _lib.load()
         half = heap_bytes // 2
         self.heaps = [_Heap(ctx, half), _Heap(ctx, half)]
         self.heap = self.heaps[0]
+        self.grow = grow
+        self.reserve = int(total * reserve_fraction)
+        self.growths = 0
         self.stage = _HostStage(ctx)  # uploads, on ctx's stream
         self.dctx = Context(getattr(ctx, "device", 0))
         self.dstage = _HostStage(self.dctx)  # deliveries, on the second stream
@@ -243,6 +260,23 @@ class Engine:
             h.mem.free()
 
     # ---- admission ----
+    def admission_limit(self, half: int) -> int:
+        """Device bytes a tick on heap half `half` may take: its size, or,
+        when the heap may grow, its size plus the device's free memory now,
+        less the reserve (another process or context may hold the rest)."""
+        cap = self.heaps[half].cap
+        if not self.grow:
+            return cap
+        free, _ = mem_info()
+        return cap + max(0, free - self.reserve)
+
+    def fit_half(self, half: int, need: int) -> None:
+        """Grows heap half `half` to hold need bytes (between ticks)."""
+        h = self.heaps[half]
+        if need > h.cap:
+            h.grow(max(need, min(2 * h.cap, self.admission_limit(half))))
+            self.growths += 1
+
     def cost(self, r: Request) -> int:
         """Device bytes the request takes in a tick (with alignment slack)."""
         if r.kind == DECOMPRESS:
@@ -256,7 +290,9 @@ class Engine:
         slot = batch.SLOT64 if v.dtype == np.uint64 else batch.SLOT32
         return v.nbytes + nb * (2 * slot + 8 + 4 + 4 + 16) + 64 + 12 * _ALIGN
 
-    def check_request(self, r: Request) -> None:
+    def check_request(self, r: Request, limit: int | None = None) -> None:
+        """Rejects a request outside its stream, or larger than the device
+        memory the server may use (limit: admission_limit(0) by default)."""
         if r.kind == DECOMPRESS:
             st = self.store.streams[r.stream]
             if r.nblocks < 1 or r.first < 0 or r.first + r.nblocks > st.nblocks:
@@ -268,8 +304,8 @@ class Engine:
                 raise ValueError("get position outside the stream")
         elif r.values.size == 0:
             raise ValueError("compress of no values")
-        if self.cost(r) > self.heap.cap:
-            raise ValueError("request larger than the serving heap")
+        if self.cost(r) > (self.admission_limit(0) if limit is None else limit):
+            raise ValueError("request larger than the device memory the server may use")
 
     # ---- one tick ----
     def run_batch(self, reqs: list[Request], deliver: bool = True) -> None:
@@ -420,8 +456,9 @@ class Engine:
         """Serves requests by arrival time (seconds from the call). Returns
         latency and throughput statistics."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        limit0 = self.admission_limit(0)
         for r in reqs:
-            self.check_request(r)
+            self.check_request(r, limit0)
         t0 = clock()
         i = tick = 0
         sizes = []
@@ -435,36 +472,47 @@ class Engine:
             for r in b:
                 r.done, r.tick = t, tick
 
-        while i < len(reqs):
-            now = clock() - t0
-            if reqs[i].arrival > now:
-                time.sleep(min(reqs[i].arrival - now, 0.001))
-                continue
-            used, b = 0, []
-            while i < len(reqs) and reqs[i].arrival <= now:
-                c = self.cost(reqs[i])
-                if b and used + c > self.heap.cap:
-                    break
-                b.append(reqs[i])
-                used += c
-                i += 1
-            half = tick & 1
-            if inflight[half] is not None:
-                inflight[half].result()  # the half's results are on the host: reuse it
-            pending = self._launch(b, half)
+        try:
+            while i < len(reqs):
+                now = clock() - t0
+                if reqs[i].arrival > now:
+                    time.sleep(min(reqs[i].arrival - now, 0.001))
+                    continue
+                half = tick & 1
+                if inflight[half] is not None:
+                    inflight[half].result()  # the half's results are on the host: reuse it
+                    inflight[half] = None
+                limit = self.admission_limit(half)
+                used, b = 0, []
+                while i < len(reqs) and reqs[i].arrival <= now:
+                    c = self.cost(reqs[i])
+                    if b and used + c > limit:
+                        break
+                    b.append(reqs[i])
+                    used += c
+                    i += 1
+                self.fit_half(half, used)
+                pending = self._launch(b, half)
+                if worker is not None:
+                    inflight[half] = worker.submit(finish, pending, b, tick)
+                else:
+                    finish([], b, tick)
+                tick += 1
+                sizes.append(len(b))
+        finally:
+            # every delivery under way completes (or fails) before serve
+            # returns or raises, so no later tick can reuse its heap half
+            for f in inflight:
+                if f is not None:
+                    f.exception()
             if worker is not None:
-                inflight[half] = worker.submit(finish, pending, b, tick)
-            else:
-                finish([], b, tick)
-            tick += 1
-            sizes.append(len(b))
+                worker.shutdown(wait=True)
         for f in inflight:
             if f is not None:
-                f.result()
-        if worker is not None:
-            worker.shutdown()
+                f.result()  # a delivery's own error
         wall = clock() - t0
-        return summarize(reqs, wall, self.store, ticks=tick, mean_batch=float(np.mean(sizes)) if sizes else 0.0)
+        return summarize(reqs, wall, self.store, ticks=tick, mean_batch=float(np.mean(sizes)) if sizes else 0.0,
+                         max_batch=int(max(sizes)) if sizes else 0, heap_growths=self.growths)
 
 
 def request_bytes(r: Request, store: Store) -> int:
