@@ -146,11 +146,13 @@ def _call_args(src, start):
 
 
 def test_go_binding_matches_the_header():
-    """The cgo package written out in INTEGRATION.md (no Go toolchain here to
-    build it) calls only functions the header declares, each with the
-    declared number of arguments, and uses only macros the header defines."""
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    go = "\n".join(re.findall(r"```go\n(.*?)```", text, flags=re.S))
+    """The cgo package in go/rocm (no Go toolchain here, nor on the GPU
+    machines, to build it) calls only functions the header declares, each
+    with the declared number of arguments, and uses only macros the header
+    defines."""
+    import glob
+    go = "\n".join(open(p).read() for p in sorted(glob.glob(os.path.join(ROOT, "go", "rocm", "*.go")))
+                   if not p.endswith("_test.go"))
     assert "import \"C\"" in go
     protos = _c_prototypes()
     calls = [(m.group(1), _call_args(go, m.end() - 1)) for m in re.finditer(r"\bC\.(utl_[a-z0-9_]+)\(", go)]
