@@ -401,7 +401,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="uneven",
-                    choices=["uneven", "delta64m", "plain1m", "u64", "serve", "serve-sweep", "walk"],
+                    choices=["uneven", "delta64m", "plain1m", "u64", "serve", "serve-sweep", "walk", "encode",
+                             "uneven-walk"],
                     help="uneven (200,000 streams) is the headline; the others print their own JSON lines")
     ap.add_argument("--streams", type=int, default=200_000, help="streams of the uneven workload")
     ap.add_argument("--values", type=int, default=64 << 20, help="values of the delta64m / walk workloads")
@@ -446,6 +447,11 @@ def main() -> int:
     elif args.workload == "serve":
         print(json.dumps(runbench.serving(ctx, store_bytes=int(args.store_gb * (1 << 30)), rate=args.rate,
                                           nreq=args.requests, progress=log)), flush=True)
+    elif args.workload == "encode":
+        for rec in runbench.encode_forms(ctx, progress=log):
+            print(json.dumps(rec), flush=True)
+    elif args.workload == "uneven-walk":
+        print(json.dumps(runbench.uneven(ctx, nstreams=args.streams, progress=log)), flush=True)
     elif args.workload == "serve-sweep":
         for rec in runbench.serving_sweep(ctx, store_bytes=int(args.store_gb * (1 << 30)), nreq=args.requests,
                                           progress=log):

@@ -136,6 +136,50 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
     return rec
 
 
+def encode_forms(ctx: Context, reps: int = 3, progress=None):
+    """The batched encode of the 64 Mi delta postings (flag Delta) and of the
+    200,000 uneven streams, values resident in HBM, in both forms: slots +
+    index + compaction, and direct writes with look-back offsets. Yields one
+    record per workload: each form's median wall time of batch.encode over
+    reps runs after a warm-up (the whole call: job table, kernels,
+    allocations), input GB/s, equal bytes of the two forms, and the form the
+    tuner's cache would pick."""
+    say = progress or (lambda m: None)
+    for name in ("delta64m", "uneven200k"):
+        if name == "delta64m":
+            v, lens, flags, flag = workloads.delta_postings(64 << 20), None, None, 1
+        else:
+            (v, lens, flags), flag = workloads.uneven_streams(200_000, seed=3), 0
+        dv = DeviceArray.from_numpy(v, ctx)
+        del v
+        times, ref, equal = {m: [] for m in batch.ENCODE_METHODS}, None, True
+        for rep in range(reps + 1):
+            for m in batch.ENCODE_METHODS:
+                ctx.sync()
+                t0 = time.perf_counter()
+                s = batch.encode(ctx, dv, flag=flag, stream_lengths=lens, stream_flags=flags, method=m)
+                times[m].append((time.perf_counter() - t0) * 1e3)
+                if rep == 0:
+                    h = s.bytes_host()
+                    if ref is None:
+                        ref = h
+                    else:
+                        equal &= bool(np.array_equal(h, ref))
+                    del h
+                nb, nbytes = s.nblocks, s.nbytes
+                s.free()
+        rec = {"workload": f"batched encode, {name}", "values": dv.n, "blocks": nb, "compressed_bytes": nbytes,
+               "forms_equal_bytes": equal}
+        for m in batch.ENCODE_METHODS:
+            ms = float(np.median(times[m][1:]))
+            rec[f"{m}_ms"] = round(ms, 4)
+            rec[f"{m}_input_GBps"] = round(dv.nbytes / (ms * 1e-3) / 1e9, 1)
+        rec["tuned_method"] = min(batch.ENCODE_METHODS, key=lambda m: rec[f"{m}_ms"])
+        say(f"encode {name}: {rec}")
+        dv.free()
+        yield rec
+
+
 def tune_summary(tune: dict) -> dict:
     """The tuner's pick and the best time of each (arena, waves, form)."""
     best = {}
