@@ -1,64 +1,30 @@
 // utl_encode.hip -- gfx950 encode kernels for UTL-PFOR blocks.
 //
-// One wavefront encodes one block. Thread t owns input positions 2t and
-// 2t+1 (row t >> 3, lanes 2(t & 7) and 2(t & 7)+1). The block is assembled
-// byte by byte in a per-wave LDS buffer and then written out with coalesced
-// byte stores into a fixed-size slot; a scan of the block lengths and a
-// compaction pass produce the concatenated stream.
+// Eight blocks per wavefront: the 8 threads of an aligned 8-lane group own
+// one block, and every per-block decision is a reduction over the group
+// (DPP), so the decision work of eight blocks runs as vector instructions in
+// one wavefront. Blocks are written into fixed-size slots (a scan of the
+// block lengths and a compaction pass then produce the concatenated stream)
+// or, for uint32 blocks, straight to their place in the stream, with
+// offsets found by a decoupled look-back over workgroup tiles.
 //
 // The steps follow packBlockScalar (scalar.go:126-235) exactly:
-//   FOR decision decideFORFull (for.go:95-118) from wave min/max and the
-//   8 threshold ballots of the raw values; per-lane delta with zigzag of all
-//   values on any decrease (delta.go:23-47); bit width by the integer cost
-//   model over exception counts (select_bitwidth.go:41-121; ties keep the
-//   wider width); payload words gathered from LDS (scalar.go:7-59);
-//   exceptions compacted by ballot rank into a sorted position list or a
-//   bitmap (exceptions.go:138-186) with their high bits StreamVByte-encoded
-//   ("1234" codes, control bytes first).
+//   FOR decision decideFORFull (for.go:95-118) from the group's min/max and
+//   the 8 threshold counts of the raw values; per-lane delta with zigzag of
+//   all values on any decrease (delta.go:23-47); bit width by the integer
+//   cost model over exception counts (select_bitwidth.go:41-121; ties keep
+//   the wider width); UTL payload words packed from registers
+//   (scalar.go:7-59); exceptions ranked within the group into a sorted
+//   position list or a bitmap (exceptions.go:138-186) with their high bits
+//   StreamVByte-encoded ("1234" codes, control bytes first).
 // uint64 blocks follow packUint64Scalar (uint64.go:302-333): FOR64, fit-in-32
-// or two sub-blocks with block2Len patched into the first.
+// or two sub-blocks with block2Len patched into the first, each sub-block
+// through the same uint32 steps.
 #include "utl_device.h"
 #include "utl_encode.h"
 #include "utl_format.h"
 
 namespace utl {
-
-constexpr int kEncWaves = 4;        // wavefronts per workgroup
-constexpr int kEncBuf = 2112;       // per-wave LDS block buffer (>= 2038)
-
-struct EncLds {
-  uint8_t blk[kEncBuf];
-  uint32_t vals[kBlock];
-  uint8_t codes[kBlock];
-};
-
-// select_bitwidth.go:56-72 chooseBestFromHist, fed with exc[si] = number of
-// values whose step index exceeds si (equivalent to the suffix sums).
-__device__ __forceinline__ void choose_best(const int* exc, int max_si, int& best_w, int& best_exc) {
-  const int bpv[9] = {1, 1, 1, 2, 2, 3, 3, 4, 4};
-  int cost = kHdr + (max_si << 6);
-  best_w = max_si << 2;
-  best_exc = 0;
-  for (int si = max_si - 1; si >= 0; --si) {
-    const int e = exc[si];
-    const int c = kHdr + (si << 6) + kSvbLen + (e < kExcBitmap ? e : kExcBitmap) + ((e + 3) >> 2) + e * bpv[max_si - si];
-    if (c < cost) {
-      cost = c;
-      best_w = si << 2;
-      best_exc = e;
-    }
-  }
-}
-
-// Counts of valid values above each step threshold 2^(4si)-1, si = 0..7.
-__device__ __forceinline__ void threshold_counts(uint32_t x0, uint32_t x1, bool v0, bool v1, int* exc) {
-#pragma unroll
-  for (int si = 0; si < 8; ++si) {
-    const uint32_t thr = (si == 0) ? 0u : ((1u << (4 * si)) - 1u);
-    exc[si] = __popcll(__ballot(v0 && x0 > thr)) + __popcll(__ballot(v1 && x1 > thr));
-  }
-  exc[8] = 0;
-}
 
 __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -66,183 +32,19 @@ __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int n) {
-  for (int i = 0; i < n; ++i) p[i] = (uint8_t)(v >> (8 * i));
-}
-
-__device__ __forceinline__ int step_index_len(uint32_t x) { return (bit_len32(x) + 3) >> 2; }
-
-// Packs one sub-block whose work values (positions >= n are 0) are x0/x1
-// into L->blk + at. Writes the header (with type_flags), svbLen, FOR base
-// (or the FOR64 base when for64 is set, the inner block being NoFOR), leaves
-// the 2-byte block2Len slot of a combine block for the caller, and returns
-// the sub-block length. x0/x1 return the work values after FOR/delta/zigzag
-// (the reference's in-place result).
-__device__ int pack_sub_block(EncLds* L, int at, uint32_t flag, uint32_t& x0, uint32_t& x1, int n,
-                              uint32_t type_flags, bool combine, bool for64, uint64_t base64) {
-  const int t = lane_id();
-  const int p0 = 2 * t, p1 = 2 * t + 1;
-  const bool v0 = p0 < n, v1 = p1 < n;
-  uint32_t hflags = type_flags | ((flag & UTL_SPECIAL) << 14);
-
-  // ---- FOR (for.go:95-118) ----
-  bool use_for = false;
-  uint32_t base = 0;
-  int fw = 0;
-  if (!(flag & UTL_NOFOR)) {
-    const uint32_t mn = uniform(wave_reduce_min(min(v0 ? x0 : 0xFFFFFFFFu, v1 ? x1 : 0xFFFFFFFFu)));
-    const uint32_t mx = uniform(wave_reduce_max(max(v0 ? x0 : 0u, v1 ? x1 : 0u)));
-    if (mn != 0) {
-      const int for_max = ((bit_len32(mx - mn) + 3) >> 2) << 2;
-      const int raw_max = ((bit_len32(mx) + 3) >> 2) << 2;
-      if (for_max < raw_max) {
-        int exc[9];
-        threshold_counts(x0, x1, v0, v1, exc);
-        int std_w, dummy;
-        choose_best(exc, step_index_len(mx), std_w, dummy);
-        if (for_max < std_w) {
-          use_for = true;
-          base = mn;
-          const int bytes = (bit_len32(mn) + 7) >> 3;
-          fw = bytes == 0 ? 0 : bytes == 1 ? 1 : bytes == 2 ? 2 : 3;
-          if (v0) x0 -= base;
-          if (v1) x1 -= base;
-          hflags |= (uint32_t)fw << 15;
-        }
-      }
-    }
-  }
-  // ---- per-lane delta (delta.go:23-47) ----
-  if (flag & UTL_DELTA) {
-    const uint32_t y0 = __shfl_up(x0, 8, 64), y1 = __shfl_up(x1, 8, 64);
-    const bool d0 = t >= 8 && v0, d1 = t >= 8 && v1;
-    const bool zz = (__ballot((d0 && x0 < y0) || (d1 && x1 < y1))) != 0;
-    if (d0) x0 -= y0;
-    if (d1) x1 -= y1;
-    if (zz) {
-      if (v0) x0 = zz_enc(x0);
-      if (v1) x1 = zz_enc(x1);
-      hflags |= kHZigZag;
-    }
-    hflags |= kHDelta;
-  }
-  // ---- bit width (select_bitwidth.go:20-121) ----
-  int bw, exc = 0;
-  const uint32_t orall = uniform(wave_reduce_or((v0 ? x0 : 0u) | (v1 ? x1 : 0u)));
-  const int max_si = step_index_len(orall);
-  if (flag & UTL_NOPATCH) {
-    bw = max_si << 2;
-  } else {
-    int cnt[9];
-    threshold_counts(x0, x1, v0, v1, cnt);
-    choose_best(cnt, max_si, bw, exc);
-  }
-  const bool has_exc = exc > 0;
-  const int forb = for64 ? kFor64 : for_base_bytes(fw);
-  const int poff = kHdr + (has_exc ? kSvbLen : 0) + forb + (combine ? kB2Len : 0);
-  const int pbytes = bw * 16;
-  uint8_t* B = L->blk + at;
-
-  // ---- payload (scalar.go:7-59) ----
-  const uint32_t m = low_mask(bw);
-  L->vals[p0] = v0 ? (x0 & m) : 0u;
-  L->vals[p1] = v1 ? (x1 & m) : 0u;
-  lds_fence();
-  const int ndw = 4 * bw;
-#pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const int j = t + 64 * half;
-    if (j < ndw) {
-      const int w = j >> 4, lane = j & 15;
-      uint32_t acc = 0;
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        const int s = r * bw - 32 * w;
-        if (s > -bw && s < 32) {
-          const uint32_t v = L->vals[16 * r + lane];
-          acc |= s >= 0 ? (v << s) : (v >> (-s));
-        }
-      }
-      put_bytes(B + poff + 4 * j, acc, 4);
-    }
-  }
-  // ---- header, FOR base ----
-  if (t == 0) {
-    put_bytes(B, encode_header(n, bw, exc, hflags | (for64 ? (3u << 15) : 0u)), 4);
-    const int boff = kHdr + (has_exc ? kSvbLen : 0);
-    if (for64) put_bytes(B + boff, base64, 8);
-    else if (use_for) put_bytes(B + boff, base, for_base_bytes(fw));
-  }
-  int len = poff + pbytes;
-  // ---- exceptions (exceptions.go:138-186) ----
-  if (has_exc) {
-    const bool e0 = v0 && x0 > m, e1 = v1 && x1 > m;
-    const uint64_t b0 = __ballot(e0), b1 = __ballot(e1);
-    const uint64_t below = (t == 0) ? 0ull : (~0ull >> (64 - t));
-    const int rank0 = __popcll(b0 & below) + __popcll(b1 & below);
-    const int rank1 = rank0 + (e0 ? 1 : 0);
-    const int exc_off = poff + pbytes;
-    const int isz = exc_index_size(exc);
-    if (exc <= kExcBitmap) {
-      if (e0) B[exc_off + rank0] = (uint8_t)p0;
-      if (e1) B[exc_off + rank1] = (uint8_t)p1;
-    } else if (t < 16) {
-      const uint32_t n0 = (uint32_t)(b0 >> (4 * t)) & 0xF, n1 = (uint32_t)(b1 >> (4 * t)) & 0xF;
-      uint32_t byte = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) byte |= (((n0 >> i) & 1) << (2 * i)) | (((n1 >> i) & 1) << (2 * i + 1));
-      B[exc_off + t] = (uint8_t)byte;
-    }
-    // StreamVByte of the high bits, in position (= rank) order
-    const uint32_t h0 = x0 >> bw, h1 = x1 >> bw;
-    const int c0 = h0 < (1u << 8) ? 0 : h0 < (1u << 16) ? 1 : h0 < (1u << 24) ? 2 : 3;
-    const int c1 = h1 < (1u << 8) ? 0 : h1 < (1u << 16) ? 1 : h1 < (1u << 24) ? 2 : 3;
-    const int l0 = e0 ? c0 + 1 : 0, l1 = e1 ? c1 + 1 : 0;
-    const uint32_t incl = wave_incl_scan((uint32_t)(l0 + l1));
-    const int total = (int)uniform(__shfl(incl, 63, 64));
-    const int ctrl = svb_ctrl_count(exc);
-    const int svb_off = exc_off + isz;
-    const int d0 = svb_off + ctrl + (int)incl - (l0 + l1);
-    if (e0) {
-      L->codes[rank0] = (uint8_t)c0;
-      put_bytes(B + d0, h0, l0);
-    }
-    if (e1) {
-      L->codes[rank1] = (uint8_t)c1;
-      put_bytes(B + d0 + l0, h1, l1);
-    }
-    lds_fence();
-    if (t < ctrl) {
-      uint32_t cb = 0;
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        if (4 * t + s < exc) cb |= (uint32_t)L->codes[4 * t + s] << (2 * s);
-      B[svb_off + t] = (uint8_t)cb;
-    }
-    const int svb_len = ctrl + total;
-    if (t == 0) put_bytes(B + kHdr, (uint32_t)svb_len, 2);
-    len = svb_off + svb_len;
-  }
-  lds_fence();
-  return len;
-}
-
-__device__ __forceinline__ void copy_out(const EncLds* L, int len, uint8_t* __restrict__ dst) {
-  for (int i = lane_id(); i < len; i += 64) dst[i] = L->blk[i];
-}
-
-// ---- uint32 blocks, eight per wavefront ----
+// ---- blocks, eight per wavefront ----
 //
 // Thread j (0..7) of a block's 8-lane group owns UTL lanes 2j and 2j+1, i.e.
 // the 16 values at positions 16r + 2j + e, so per-lane delta runs in
 // registers and the payload words of its two lanes are packed from
-// registers. Every per-block decision is a group reduction (DPP), so the
-// decision work of 8 blocks runs as vector instructions in one wavefront.
+// registers.
 //
-// Slot layout: the block starts at slot byte sh = (-poff) & 3, which puts
-// its payload (and everything after it) on a 4-byte boundary, so all stores
-// are aligned dwords. The bytes before the block are 0 and the block's first
-// byte (its count) is not, which is how the compaction finds sh.
+// Slot layout: a uint32 block starts at slot byte sh = (-poff) & 3, which
+// puts its payload (and everything after it) on a 4-byte boundary, so all
+// stores are aligned dwords; the bytes before it are 0. A uint64 block
+// starts at slot byte 0 (sh = 0) and is written with the unaligned writers
+// below, as its second sub-block starts at any byte. A block's first byte
+// (its count) is never 0, which is how the compaction finds sh.
 
 constexpr int kEncGrpWaves = 4;
 constexpr int kWaveBlocksEnc = 8;  // blocks per wavefront
@@ -301,6 +103,172 @@ __device__ __forceinline__ void choose_best_g(uint32_t lo, uint32_t hi, int max_
       best_exc = e;
     }
   }
+}
+
+// ---- the steps of one (sub-)block, per 8-lane group ----
+// x0/x1 hold the thread's 16 work values; positions at or past the count
+// (ok0/ok1 false) hold 0 on entry and keep 0.
+
+// FOR decision decideFORFull (for.go:95-118), given the min mn and max mx of
+// the valid values: when FOR pays, subtracts the base from the valid values
+// and returns the FOR width code (1, 2, 3: a 1, 2 or 4 byte base); 0 = no FOR.
+__device__ __forceinline__ int enc_for(uint32_t* x0, uint32_t* x1, const bool* ok0, const bool* ok1, uint32_t mn,
+                                       uint32_t mx, uint32_t& base) {
+  base = 0;
+  if (mn == 0) return 0;
+  const int for_max = ((bit_len32(mx - mn) + 3) >> 2) << 2;
+  const int raw_max = ((bit_len32(mx) + 3) >> 2) << 2;
+  if (for_max >= raw_max) return 0;
+  uint32_t lo, hi;
+  step_counts(x0, x1, lo, hi);
+  int std_w, unused;
+  choose_best_g(lo, hi, (int)step_len(mx), std_w, unused);
+  if (for_max >= std_w) return 0;
+  base = mn;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    x0[r] = ok0[r] ? x0[r] - base : 0u;
+    x1[r] = ok1[r] ? x1[r] - base : 0u;
+  }
+  const int bytes = (bit_len32(mn) + 7) >> 3;  // >= 1
+  return bytes == 1 ? 1 : bytes == 2 ? 2 : 3;
+}
+
+// Per-lane delta (delta.go:23-47): row r minus row r-1 in each UTL lane, and
+// zigzag of every value when any valid one decreased. Returns the header bits.
+__device__ __forceinline__ uint32_t enc_delta(uint32_t* x0, uint32_t* x1, const bool* ok0, const bool* ok1) {
+  bool dec = false;
+#pragma unroll
+  for (int r = kRows - 1; r >= 1; --r) {
+    dec |= ok0[r] && x0[r] < x0[r - 1];
+    dec |= ok1[r] && x1[r] < x1[r - 1];
+    x0[r] = ok0[r] ? x0[r] - x0[r - 1] : 0u;
+    x1[r] = ok1[r] ? x1[r] - x1[r - 1] : 0u;
+  }
+  if (!g8_or(dec ? 1u : 0u)) return kHDelta;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    x0[r] = zz_enc(x0[r]);  // zz_enc(0) = 0 keeps invalid positions at 0
+    x1[r] = zz_enc(x1[r]);
+  }
+  return kHDelta | kHZigZag;
+}
+
+// Bit width and exception count (select_bitwidth.go:20-121): the widest step
+// with NoPatch, else the cost model.
+__device__ __forceinline__ void enc_width(const uint32_t* x0, const uint32_t* x1, bool nopatch, int& bw, int& exc) {
+  uint32_t orall = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) orall |= x0[r] | x1[r];
+  const int max_si = (int)step_len(g8_or(orall));
+  bw = max_si << 2;
+  exc = 0;
+  if (!nopatch) {
+    uint32_t lo, hi;
+    step_counts(x0, x1, lo, hi);
+    choose_best_g(lo, hi, max_si, bw, exc);
+  }
+}
+
+// Exception section (exceptions.go:138-186) of a block with exc > 0
+// exceptions at width bw, assembled in L->tail[blk]: the position index
+// (sorted position list up to kExcBitmap exceptions, else the 16-byte
+// bitmap), then the StreamVByte control bytes, then the data bytes (the high
+// bits of each exception, in position order). Returns svbLen (control +
+// data bytes); the section is exc_index_size(exc) + svbLen bytes.
+__device__ __forceinline__ int enc_tail(EncGrpLds* L, int blk, int j, const uint32_t* x0, const uint32_t* x1, const bool* ok0,
+                        const bool* ok1, int bw, int exc) {
+  const uint32_t m = low_mask(bw);
+  const int isz = exc_index_size(exc);
+  // bitmap of exception positions (word d: rows 2d, 2d+1) and ranks
+  uint32_t bm[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int bit = 16 * (r & 1) + 2 * j;
+    bm[r >> 1] |= ((ok0[r] && x0[r] > m) ? 1u : 0u) << bit;
+    bm[r >> 1] |= ((ok1[r] && x1[r] > m) ? 1u : 0u) << (bit + 1);
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d]);
+  const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
+  uint8_t* lens = L->len[blk];
+  // data bytes of each exception, by rank
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
+    const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
+    const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
+    const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
+    if (e0) lens[rank0] = (uint8_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1);
+    if (e1) lens[rank0 + (e0 ? 1 : 0)] = (uint8_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1);
+  }
+  lds_fence();
+  // thread j: data offsets and control bytes of ranks 16j .. 16j+15
+  const int nvalid = min(max(exc - 16 * j, 0), 16);
+  const uint4 lq = *reinterpret_cast<const uint4*>(&lens[16 * j]);
+  uint32_t lw[4] = {lq.x, lq.y, lq.z, lq.w};
+  uint32_t tot = 0;
+  uint32_t incl4[4], ctrlb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int kv = min(max(nvalid - 4 * k, 0), 4);  // valid ranks in this dword
+    const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
+    const uint32_t l = lw[k] & keep;
+    // codes: length - 1 in 2-bit fields, only for valid ranks
+    const uint32_t c = ((l | ~keep) - 0x01010101u) & keep;  // bytes 0..3
+    ctrlb[k] = (c & 3u) | ((c >> 6) & 0xCu) | ((c >> 12) & 0x30u) | ((c >> 18) & 0xC0u);
+    incl4[k] = l * 0x01010101u + (tot * 0x01010101u);  // inclusive byte prefix + earlier dwords
+    tot += (l * 0x01010101u) >> 24;
+  }
+  const uint32_t scan = g8_incl_scan(tot);
+  const uint32_t tbase = scan - tot;
+  const uint32_t total = (uint32_t)__shfl((int)scan, lane_id() | 7, 64);
+  const int ctrl = svb_ctrl_count(exc);
+  uint16_t* offs = L->off[blk];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int kv = min(max(nvalid - 4 * k, 0), 4);
+    const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
+    const uint32_t l = lw[k] & keep;
+    const uint32_t excl = incl4[k] - l;  // exclusive byte prefix (bytes stay <= 64)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) offs[16 * j + 4 * k + s] = (uint16_t)(tbase + ((excl >> (8 * s)) & 0xFF));
+    if (4 * j + k < ctrl) L->tail[blk][isz + 4 * j + k] = (uint8_t)ctrlb[k];
+  }
+  lds_fence();
+  // exception data (high bits, little endian) and the position index
+  uint8_t* tl = L->tail[blk];
+  const int dbase = isz + ctrl;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
+    const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
+    const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
+    const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
+    const uint32_t rank1 = rank0 + (e0 ? 1u : 0u);
+    if (e0) {
+      const uint32_t h = x0[r] >> bw;
+      const int o = dbase + offs[rank0], l = lens[rank0];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
+      if (exc <= kExcBitmap) tl[rank0] = (uint8_t)(16 * r + 2 * j);
+    }
+    if (e1) {
+      const uint32_t h = x1[r] >> bw;
+      const int o = dbase + offs[rank1], l = lens[rank1];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
+      if (exc <= kExcBitmap) tl[rank1] = (uint8_t)(16 * r + 2 * j + 1);
+    }
+  }
+  if (exc > kExcBitmap) {  // bitmap bytes 2j, 2j+1 (position 8q + i is bit i of byte q)
+    tl[2 * j] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1)));
+    tl[2 * j + 1] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1) + 8));
+  }
+  lds_fence();
+  return ctrl + (int)total;
 }
 
 // UTL pack of lanes 2j, 2j+1 (scalar.go:7-59): lane L's word w goes to
@@ -634,64 +602,12 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
     }
   }
   uint32_t hflags = kTypeU32 | ((flag & UTL_SPECIAL) << 14);
-
-  // ---- FOR (for.go:95-118) ----
-  int fw = 0;
   uint32_t base = 0;
-  mn = g8_min(mn);
-  mx = g8_max(mx);
-  if (!(flag & UTL_NOFOR) && mn != 0) {
-    const int for_max = ((bit_len32(mx - mn) + 3) >> 2) << 2;
-    const int raw_max = ((bit_len32(mx) + 3) >> 2) << 2;
-    if (for_max < raw_max) {
-      uint32_t lo, hi;
-      step_counts(x0, x1, lo, hi);
-      int std_w, unused;
-      choose_best_g(lo, hi, (int)step_len(mx), std_w, unused);
-      if (for_max < std_w) {
-        base = mn;
-        const int bytes = (bit_len32(mn) + 7) >> 3;
-        fw = bytes == 0 ? 0 : bytes == 1 ? 1 : bytes == 2 ? 2 : 3;
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-          x0[r] = ok0[r] ? x0[r] - base : 0u;
-          x1[r] = ok1[r] ? x1[r] - base : 0u;
-        }
-        hflags |= (uint32_t)fw << 15;
-      }
-    }
-  }
-  // ---- per-lane delta (delta.go:23-47) ----
-  if (flag & UTL_DELTA) {
-    bool dec = false;
-#pragma unroll
-    for (int r = kRows - 1; r >= 1; --r) {
-      dec |= ok0[r] && x0[r] < x0[r - 1];
-      dec |= ok1[r] && x1[r] < x1[r - 1];
-      x0[r] = ok0[r] ? x0[r] - x0[r - 1] : 0u;
-      x1[r] = ok1[r] ? x1[r] - x1[r - 1] : 0u;
-    }
-    if (g8_or(dec ? 1u : 0u)) {
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        x0[r] = zz_enc(x0[r]);  // zz_enc(0) = 0 keeps invalid positions at 0
-        x1[r] = zz_enc(x1[r]);
-      }
-      hflags |= kHZigZag;
-    }
-    hflags |= kHDelta;
-  }
-  // ---- bit width (select_bitwidth.go:20-121) ----
-  uint32_t orall = 0;
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) orall |= x0[r] | x1[r];
-  const int max_si = (int)step_len(g8_or(orall));
-  int bw = max_si << 2, exc = 0;
-  if (!(flag & UTL_NOPATCH)) {
-    uint32_t lo, hi;
-    step_counts(x0, x1, lo, hi);
-    choose_best_g(lo, hi, max_si, bw, exc);
-  }
+  const int fw = (flag & UTL_NOFOR) ? 0 : enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), base);
+  hflags |= (uint32_t)fw << 15;
+  if (flag & UTL_DELTA) hflags |= enc_delta(x0, x1, ok0, ok1);
+  int bw, exc;
+  enc_width(x0, x1, (flag & UTL_NOPATCH) != 0, bw, exc);
   if (work_out && ok) {
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
@@ -745,105 +661,13 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
   }
 
   // ---- exceptions (exceptions.go:138-186) ----
-  int svb_len = 0;
   const int isz = exc_index_size(exc);
-  if (has_exc) {
-    // bitmap of exception positions (word d: rows 2d, 2d+1) and ranks
-    uint32_t bm[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      const int bit = 16 * (r & 1) + 2 * j;
-      bm[r >> 1] |= ((ok0[r] && x0[r] > m) ? 1u : 0u) << bit;
-      bm[r >> 1] |= ((ok1[r] && x1[r] > m) ? 1u : 0u) << (bit + 1);
-    }
-#pragma unroll
-    for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d]);
-    const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
-    uint8_t* lens = L->len[blk];
-    // data bytes of each exception, by rank
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
-      const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-      const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
-      const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
-      if (e0) lens[rank0] = (uint8_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1);
-      if (e1) lens[rank0 + (e0 ? 1 : 0)] = (uint8_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1);
-    }
-    lds_fence();
-    // thread j: data offsets and control bytes of ranks 16j .. 16j+15
-    const int nvalid = min(max(exc - 16 * j, 0), 16);
-    const uint4 lq = *reinterpret_cast<const uint4*>(&lens[16 * j]);
-    uint32_t lw[4] = {lq.x, lq.y, lq.z, lq.w};
-    uint32_t tot = 0;
-    uint32_t incl4[4], ctrlb[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int kv = min(max(nvalid - 4 * k, 0), 4);  // valid ranks in this dword
-      const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
-      const uint32_t l = lw[k] & keep;
-      // codes: length - 1 in 2-bit fields, only for valid ranks
-      const uint32_t c = ((l | ~keep) - 0x01010101u) & keep;  // bytes 0..3
-      ctrlb[k] = (c & 3u) | ((c >> 6) & 0xCu) | ((c >> 12) & 0x30u) | ((c >> 18) & 0xC0u);
-      incl4[k] = l * 0x01010101u + (tot * 0x01010101u);  // inclusive byte prefix + earlier dwords
-      tot += (l * 0x01010101u) >> 24;
-    }
-    const uint32_t scan = g8_incl_scan(tot);
-    const uint32_t tbase = scan - tot;
-    const uint32_t total = (uint32_t)__shfl((int)scan, lane_id() | 7, 64);
-    const int ctrl = svb_ctrl_count(exc);
-    uint16_t* offs = L->off[blk];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int kv = min(max(nvalid - 4 * k, 0), 4);
-      const uint32_t keep = kv >= 4 ? 0xFFFFFFFFu : ((1u << (8 * kv)) - 1u);
-      const uint32_t l = lw[k] & keep;
-      const uint32_t excl = incl4[k] - l;  // exclusive byte prefix (bytes stay <= 64)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) offs[16 * j + 4 * k + s] = (uint16_t)(tbase + ((excl >> (8 * s)) & 0xFF));
-      if (4 * j + k < ctrl) L->tail[blk][isz + 4 * j + k] = (uint8_t)ctrlb[k];
-    }
-    lds_fence();
-    // exception data (high bits, little endian) and the position index
-    uint8_t* tl = L->tail[blk];
-    const int dbase = isz + ctrl;
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
-      const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-      const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
-      const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
-      const uint32_t rank1 = rank0 + (e0 ? 1u : 0u);
-      if (e0) {
-        const uint32_t h = x0[r] >> bw;
-        const int o = dbase + offs[rank0], l = lens[rank0];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
-        if (exc <= kExcBitmap) tl[rank0] = (uint8_t)(16 * r + 2 * j);
-      }
-      if (e1) {
-        const uint32_t h = x1[r] >> bw;
-        const int o = dbase + offs[rank1], l = lens[rank1];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (q < l) tl[o + q] = (uint8_t)(h >> (8 * q));
-        if (exc <= kExcBitmap) tl[rank1] = (uint8_t)(16 * r + 2 * j + 1);
-      }
-    }
-    if (exc > kExcBitmap) {  // bitmap bytes 2j, 2j+1 (position 8q + i is bit i of byte q)
-      tl[2 * j] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1)));
-      tl[2 * j + 1] = (uint8_t)(bm[j >> 1] >> (16 * (j & 1) + 8));
-    }
-    svb_len = ctrl + (int)total;
-    lds_fence();
-    // copy the tail (aligned in the slot right after the payload)
-    if (!DIRECT && ok) {
-      uint32_t* dst = pay + 4 * bw;
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(tl);
-      const int ndw = (isz + svb_len + 3) >> 2;
-      for (int k = j; k < ndw; k += 8) dst[k] = src[k];
-    }
+  const int svb_len = has_exc ? enc_tail(L, blk, j, x0, x1, ok0, ok1, bw, exc) : 0;
+  if (has_exc && !DIRECT && ok) {  // the tail, aligned in the slot right after the payload
+    uint32_t* dst = pay + 4 * bw;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(L->tail[blk]);
+    const int ndw = (isz + svb_len + 3) >> 2;
+    for (int k = j; k < ndw; k += 8) dst[k] = src[k];
   }
   const int len = poff + 16 * bw + (has_exc ? isz + svb_len : 0);
   if (DIRECT) {
@@ -906,55 +730,184 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32(const uint32_t
   }
 }
 
-// uint64.go:302-333 packUint64Scalar
-__global__ __launch_bounds__(64 * kEncWaves) void k_encode_u64(const uint64_t* __restrict__ values,
-                                                              const uint64_t* __restrict__ in_off,
-                                                              const uint32_t* __restrict__ in_meta, uint64_t nblocks,
-                                                              uint8_t* __restrict__ slots, uint32_t slot_bytes,
-                                                              uint32_t* __restrict__ out_meta) {
-  __shared__ EncLds lds[kEncWaves];
-  const uint32_t wave = uniform(threadIdx.x >> 6);
-  const uint64_t b = (uint64_t)blockIdx.x * kEncWaves + wave;
-  if (b >= nblocks) return;
-  EncLds* L = &lds[wave];
-  const uint32_t meta = in_meta[b];
-  const int n = (int)(meta & 0xFF);
-  const uint32_t flag = (meta >> 8) & 0xFF;
-  const int t = lane_id();
-  if (n < 1 || n > kBlock) {  // packUint64Scalar: ErrInvalidBuffer (uint64.go:304-306)
-    if (t == 0) out_meta[b] = 0xFFFFFFFFu;
-    return;
+// ---- uint64 blocks, eight per wavefront ----
+
+template <int Ctrl>
+__device__ __forceinline__ uint64_t g8_dpp64(uint64_t x) {
+  return ((uint64_t)g8_dpp<Ctrl>((uint32_t)(x >> 32)) << 32) | g8_dpp<Ctrl>((uint32_t)x);
+}
+__device__ __forceinline__ uint64_t g8_min64(uint64_t x) {
+  x = min(x, g8_dpp64<0xB1>(x));
+  x = min(x, g8_dpp64<0x4E>(x));
+  return min(x, g8_dpp64<0x141>(x));
+}
+__device__ __forceinline__ uint64_t g8_max64(uint64_t x) {
+  x = max(x, g8_dpp64<0xB1>(x));
+  x = max(x, g8_dpp64<0x4E>(x));
+  return max(x, g8_dpp64<0x141>(x));
+}
+__device__ __forceinline__ uint64_t g8_or64(uint64_t x) {
+  return ((uint64_t)g8_or((uint32_t)(x >> 32)) << 32) | g8_or((uint32_t)x);
+}
+
+// A sub-block's fields before its payload: header, svbLen (with
+// exceptions), FOR base (fbb bytes: 0, 1, 2, 4, or 8 for FOR64), block2Len
+// (the first sub-block of a combine block).
+struct SubHead {
+  uint32_t hw, svb_len, b2len;
+  uint64_t base;
+  int fbb, poff;
+  bool has_exc;
+};
+
+__device__ __forceinline__ uint32_t head_byte(const SubHead& h, int k) {
+  if (k < 4) return (h.hw >> (8 * k)) & 0xFFu;
+  k -= 4;
+  if (h.has_exc) {
+    if (k < 2) return (h.svb_len >> (8 * k)) & 0xFFu;
+    k -= 2;
   }
-  const uint64_t* v = values + in_off[b];
-  const bool v0 = 2 * t < n, v1 = 2 * t + 1 < n;
-  const uint64_t a0 = v0 ? v[2 * t] : 0ull, a1 = v1 ? v[2 * t + 1] : 0ull;
-  const uint64_t mn = wave_reduce_min64(min(v0 ? a0 : ~0ull, v1 ? a1 : ~0ull));
-  const uint64_t mx = wave_reduce_max64(max(a0, a1));
-  const uint64_t orall = wave_reduce_or64(a0 | a1);
-  int len;
-  if (!(flag & UTL_NOFOR) && mn > 0 && (mx - mn) < (1ull << 32)) {
-    uint32_t x0 = (uint32_t)(a0 - mn), x1 = (uint32_t)(a1 - mn);
-    if (!v0) x0 = 0;
-    if (!v1) x1 = 0;
-    len = pack_sub_block(L, 0, flag | UTL_NOFOR, x0, x1, n, kTypeU64, false, true, mn);
-  } else if ((orall >> 32) == 0) {
-    uint32_t x0 = (uint32_t)a0, x1 = (uint32_t)a1;
-    len = pack_sub_block(L, 0, flag, x0, x1, n, kTypeU64, false, false, 0);
-  } else {
-    uint32_t l0 = (uint32_t)a0, l1 = (uint32_t)a1, h0 = (uint32_t)(a0 >> 32), h1 = (uint32_t)(a1 >> 32);
-    const int len1 = pack_sub_block(L, 0, flag, l0, l1, n, kTypeU64 | kHCombine, true, false, 0);
-    const int len2 = pack_sub_block(L, len1, flag, h0, h1, n, kTypeU64, false, false, 0);
-    if (t == 0) {
-      const uint32_t hd = (uint32_t)L->blk[0] | ((uint32_t)L->blk[1] << 8) | ((uint32_t)L->blk[2] << 16) |
-                          ((uint32_t)L->blk[3] << 24);
-      const Hdr h = decode_header(hd);
-      put_bytes(L->blk + kHdr + (h.exc > 0 ? kSvbLen : 0) + for_base_bytes(h.for_width), (uint32_t)len2, 2);
+  if (k < h.fbb) return (uint32_t)(h.base >> (8 * k)) & 0xFFu;
+  return (h.b2len >> (8 * (k - h.fbb))) & 0xFFu;
+}
+
+// Bytes [0, poff) of a sub-block at byte o of g (poff <= 14): two per thread.
+__device__ __forceinline__ void put_head_at(const SubHead& h, uint8_t* __restrict__ g, uint64_t o, int j) {
+  if (j < h.poff) g[o + j] = (uint8_t)head_byte(h, j);
+  if (j + 8 < h.poff) g[o + j + 8] = (uint8_t)head_byte(h, j + 8);
+}
+
+// One 32-bit sub-block of a uint64 block (packBlockScalar's steps, as for a
+// uint32 block) with header type bits tflags: its payload and exception
+// section written at byte o of g when `write`; its header fields returned
+// (written by the caller, which knows block2Len last). for64: the values are
+// already minus the FOR64 base `base64` and the sub-block carries it.
+__device__ __forceinline__ SubHead enc_sub_u64(EncGrpLds* L, int blk, int j, uint32_t* x0, uint32_t* x1,
+                                               const bool* ok0, const bool* ok1, int n, uint32_t flag,
+                                               uint32_t tflags, bool for64, uint64_t base64, bool combine,
+                                               uint8_t* __restrict__ g, uint64_t o, bool write) {
+  uint32_t hflags = tflags | ((flag & UTL_SPECIAL) << 14);
+  int fw = 0;
+  uint32_t base = 0;
+  if (!for64 && !(flag & UTL_NOFOR)) {
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      if (ok0[r]) {
+        mn = min(mn, x0[r]);
+        mx = max(mx, x0[r]);
+      }
+      if (ok1[r]) {
+        mn = min(mn, x1[r]);
+        mx = max(mx, x1[r]);
+      }
     }
-    lds_fence();
-    len = len1 + len2;
+    fw = enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), base);
   }
-  copy_out(L, len, slots + b * slot_bytes);
-  if (t == 0) out_meta[b] = (uint32_t)len | ((uint32_t)n << 24);
+  hflags |= (for64 ? 3u : (uint32_t)fw) << 15;
+  if (flag & UTL_DELTA) hflags |= enc_delta(x0, x1, ok0, ok1);
+  int bw, exc;
+  enc_width(x0, x1, (flag & UTL_NOPATCH) != 0, bw, exc);
+  SubHead h;
+  h.has_exc = exc > 0;
+  h.hw = encode_header(n, bw, exc, hflags);
+  h.base = for64 ? base64 : base;
+  h.fbb = for64 ? kFor64 : for_base_bytes(fw);
+  h.b2len = 0;
+  h.poff = kHdr + (h.has_exc ? kSvbLen : 0) + h.fbb + (combine ? kB2Len : 0);
+  const int isz = exc_index_size(exc);
+  h.svb_len = h.has_exc ? (uint32_t)enc_tail(L, blk, j, x0, x1, ok0, ok1, bw, exc) : 0u;
+  if (write) {
+    const uint32_t m = low_mask(bw);
+    uint32_t p0[kRows], p1[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      p0[r] = x0[r] & m;
+      p1[r] = x1[r] & m;
+    }
+    pack_any_at(bw, p0, p1, g, o + h.poff, j);
+    if (h.has_exc) copy_lds_at(L->tail[blk], isz + (int)h.svb_len, g, o + h.poff + 16 * bw, j);
+  }
+  return h;
+}
+
+__device__ __forceinline__ int sub_len(const SubHead& h) {
+  const int bw = (int)((h.hw >> 8) & 0x1F) << 2, exc = (int)(h.hw >> 24);
+  return h.poff + 16 * bw + (h.has_exc ? exc_index_size(exc) + (int)h.svb_len : 0);
+}
+
+// Encodes the blocks of a u64 job table into slots, eight per wavefront,
+// following packUint64Scalar (uint64.go:302-333): FOR64 when the minimum is
+// above 0 and max - min < 2^32 (one sub-block of value - min, flag NoFOR,
+// the 8-byte base); fit-32 when every value is below 2^32 (one sub-block);
+// else combine: the low halves, then the high halves, each a sub-block with
+// the block's flag, the first carrying the second's length. Block b starts
+// at byte 0 of slot b.
+__global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u64(const uint64_t* __restrict__ values,
+                                                                  const uint64_t* __restrict__ in_off,
+                                                                  const uint32_t* __restrict__ in_meta,
+                                                                  uint64_t nblocks, uint8_t* __restrict__ slots,
+                                                                  uint32_t slot_bytes, uint32_t* __restrict__ out_meta) {
+  __shared__ EncGrpLds lds[kEncGrpWaves];
+  const uint32_t wave = uniform(threadIdx.x >> 6);
+  const uint64_t b0 = ((uint64_t)blockIdx.x * kEncGrpWaves + wave) * kWaveBlocksEnc;
+  if (b0 >= nblocks) return;
+  EncGrpLds* L = &lds[wave];
+  const int blk = lane_id() >> 3, j = lane_id() & 7;
+  const bool mine = b0 + blk < nblocks;
+  const uint64_t b = mine ? b0 + blk : b0;  // a lane without a block reads block b0's job
+  const uint32_t meta = in_meta[b];
+  const uint32_t flag = (meta >> 8) & 0xFF;
+  const int n_req = (int)(meta & 0xFF);
+  const bool ok = mine && n_req >= 1 && n_req <= kBlock;  // else ErrInvalidBuffer (uint64.go:304-306)
+  const int n = ok ? n_req : 0;
+  const uint64_t* v = values + in_off[b];
+
+  uint64_t a0[kRows], a1[kRows];
+  bool ok0[kRows], ok1[kRows];
+  uint64_t mn = ~0ull, mx = 0, orv = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = 16 * r + 2 * j;
+    ok0[r] = p < n;
+    ok1[r] = p + 1 < n;
+    a0[r] = ok0[r] ? v[p] : 0ull;
+    a1[r] = ok1[r] ? v[p + 1] : 0ull;
+    if (ok0[r]) mn = min(mn, a0[r]);
+    if (ok1[r]) mn = min(mn, a1[r]);
+    mx = max(mx, max(a0[r], a1[r]));
+    orv |= a0[r] | a1[r];
+  }
+  mn = g8_min64(mn);
+  mx = g8_max64(mx);
+  orv = g8_or64(orv);
+  const bool for64 = !(flag & UTL_NOFOR) && mn > 0 && mx - mn < (1ull << 32);
+  const bool combine = !for64 && (orv >> 32) != 0;
+  uint32_t x0[kRows], x1[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    x0[r] = ok0[r] ? (uint32_t)(for64 ? a0[r] - mn : a0[r]) : 0u;
+    x1[r] = ok1[r] ? (uint32_t)(for64 ? a1[r] - mn : a1[r]) : 0u;
+  }
+  const uint64_t o = b * slot_bytes;
+  SubHead h1 = enc_sub_u64(L, blk, j, x0, x1, ok0, ok1, n, for64 ? flag | UTL_NOFOR : flag,
+                           kTypeU64 | (combine ? kHCombine : 0u), for64, mn, combine, slots, o, ok);
+  int len = sub_len(h1);
+  if (combine && ok) {  // the groups of combine blocks: the high halves after the first sub-block
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      x0[r] = (uint32_t)(a0[r] >> 32);
+      x1[r] = (uint32_t)(a1[r] >> 32);
+    }
+    lds_fence();  // the first sub-block's exception bytes are written out
+    const SubHead h2 = enc_sub_u64(L, blk, j, x0, x1, ok0, ok1, n, flag, kTypeU64, false, 0, false, slots,
+                                   o + len, true);
+    put_head_at(h2, slots, o + len, j);
+    h1.b2len = (uint32_t)sub_len(h2);
+    len += (int)h1.b2len;
+  }
+  if (ok) put_head_at(h1, slots, o, j);
+  if (j == 0 && mine) out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
 }
 
 // Concatenates the slots into dst (block b's bytes at dst + boff[b]) with
@@ -1092,9 +1045,10 @@ hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  hipStream_t stream) {
   if (nblocks == 0) return hipSuccess;
-  const uint64_t grid = (nblocks + kEncWaves - 1) / kEncWaves;
-  k_encode_u64<<<dim3((unsigned)grid), dim3(64 * kEncWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
-                                                                            slot_bytes, out_meta);
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  const uint64_t grid = (nblocks + per_wg - 1) / per_wg;
+  k_encode_u64<<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
+                                                                               slot_bytes, out_meta);
   return hipGetLastError();
 }
 
