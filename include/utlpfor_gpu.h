@@ -299,8 +299,7 @@ int utl_encode_u32_direct(utl_ctx *ctx, const uint32_t *d_values, const uint64_t
                           uint32_t *d_out_meta, uint64_t *d_block_off, size_t *out_nbytes);
 /* Concatenates encoded slots into d_dst at d_block_off (from utl_index_build
  * over d_out_meta): byte-identical to appending the blocks with Append.
- * Writes whole aligned dwords: d_dst must hold d_block_off[nblocks] bytes
- * rounded up to a multiple of 4 (the bytes past the stream are unspecified). */
+ * Writes exactly the stream's d_block_off[nblocks] bytes. */
 int utl_compact(utl_ctx *ctx, int is_u64, const uint8_t *d_slots, const uint64_t *d_block_off, size_t nblocks,
                 uint8_t *d_dst);
 /* Job table for streams cut into 128-value blocks (the last one partial):

@@ -685,8 +685,8 @@ int utl_compact(utl_ctx* ctx, int is_u64, const uint8_t* d_slots, const uint64_t
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_compact(d_slots, is_u64 ? UTL_SLOT_BYTES64 : UTL_SLOT_BYTES32, d_block_off, nblocks, d_dst,
-                             c->stream));
+  HIP_TRY(utl_launch_compact(d_slots, is_u64 ? UTL_SLOT_BYTES64 : UTL_SLOT_BYTES32, !is_u64, d_block_off, nblocks,
+                             d_dst, c->stream));
   return UTL_OK;
 }
 

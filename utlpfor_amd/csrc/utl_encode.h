@@ -5,8 +5,9 @@
 #include <stdint.h>
 
 // Job table: block b encodes values[in_off[b] .. +count) with
-// in_meta[b] = count (1..128) | flag << 8. Each block is written into slot
-// b (slot_bytes apart); out_meta[b] = length | count << 24.
+// in_meta[b] = count (1..128) | flag << 8; out_meta[b] = length | count << 24.
+// uint32: the blocks of tile t (blocks 32t .. 32t + 31) are written back to
+// back from byte 32t * slot_bytes; uint64: block b at byte b * slot_bytes.
 hipError_t utl_launch_encode_u32(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  uint32_t* work_out, hipStream_t stream);
@@ -23,7 +24,9 @@ hipError_t utl_launch_encode_u32_direct(const uint32_t* values, const uint64_t* 
 hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  hipStream_t stream);
-hipError_t utl_launch_compact(const uint8_t* slots, uint32_t slot_bytes, const uint64_t* boff, uint64_t nblocks,
-                              uint8_t* dst, hipStream_t stream);
+// Concatenation of encoded slots: tiles (the uint32 layout) or one block per
+// slot (uint64), into dst at the block offsets boff.
+hipError_t utl_launch_compact(const uint8_t* slots, uint32_t slot_bytes, bool tiles, const uint64_t* boff,
+                              uint64_t nblocks, uint8_t* dst, hipStream_t stream);
 hipError_t utl_launch_make_jobs(const uint64_t* val_off, const uint64_t* blk_off, const uint8_t* flags,
                                 uint64_t nstreams, uint64_t* in_off, uint32_t* in_meta, hipStream_t stream);
