@@ -277,9 +277,12 @@ int utl_gather_index(utl_ctx *ctx, const uint64_t *d_block_off, const uint64_t *
 
 /* Batched PackUint32 / PackUint64 (pack.go:76, uint64.go:279) over a job
  * table: block b encodes values[d_in_off[b]..] with
- * d_in_meta[b] = count (1..128) | flag << 8 into slot b of d_slots
- * (UTL_SLOT_BYTES32/64 apart); d_out_meta[b] = length | count << 24, or
- * 0xFFFFFFFF for a count outside 1..128 (ErrInvalidBuffer). */
+ * d_in_meta[b] = count (1..128) | flag << 8 into d_slots (nblocks *
+ * UTL_SLOT_BYTES32/64 bytes): uint64 block b at byte b * UTL_SLOT_BYTES64;
+ * uint32 blocks 32t .. 32t + 31 back to back from byte
+ * 32t * UTL_SLOT_BYTES32. d_out_meta[b] = length | count << 24, or
+ * 0xFFFFFFFF for a count outside 1..128 (ErrInvalidBuffer). utl_compact
+ * turns the slots into the stream. */
 #define UTL_SLOT_BYTES32 1024u
 #define UTL_SLOT_BYTES64 2048u
 int utl_encode_blocks(utl_ctx *ctx, int is_u64, const void *d_values, const uint64_t *d_in_off,
