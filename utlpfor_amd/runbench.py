@@ -5,6 +5,7 @@ before the timed region; every run checks its decoded output against the
 generated input (bit-exact round trip through the GPU encoder)."""
 from __future__ import annotations
 
+import ctypes as C
 import time
 
 import numpy as np
@@ -175,9 +176,66 @@ def encode_forms(ctx: Context, reps: int = 3, progress=None):
             rec[f"{m}_ms"] = round(ms, 4)
             rec[f"{m}_input_GBps"] = round(dv.nbytes / (ms * 1e-3) / 1e9, 1)
         rec["tuned_method"] = min(batch.ENCODE_METHODS, key=lambda m: rec[f"{m}_ms"])
+        rec["device_ms"] = encode_device_ms(ctx, dv, lens, flags, flag, nb, reps)
+        rec["device_input_GBps"] = {m: round(dv.nbytes / (t * 1e-3) / 1e9, 1) for m, t in rec["device_ms"].items()}
         say(f"encode {name}: {rec}")
         dv.free()
         yield rec
+
+
+def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblocks: int, reps: int = 3) -> dict:
+    """GPU time of each encode form's kernels on buffers allocated once (HIP
+    events around job table + encode + index scan (+ compaction); the
+    median of reps after a warm-up): what batch.encode costs without its
+    allocations and host reads."""
+    lib = _lib.load()
+    n = dv.n
+    lens = np.array([n], np.uint64) if lens is None else np.asarray(lens, np.uint64)
+    flags = np.full(lens.size, flag, np.uint8) if flags is None else np.asarray(flags, np.uint8)
+    blk_off = batch.plan_blocks(lens)
+    val_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    d_val, d_blk, d_fl = (DeviceArray.from_numpy(a, ctx) for a in (val_off, blk_off, flags))
+    in_off, in_meta = DeviceArray(nblocks, np.uint64, ctx), DeviceArray(nblocks, np.uint32, ctx)
+    meta = DeviceArray(nblocks, np.uint32, ctx)
+    boff, ooff = DeviceArray(nblocks + 1, np.uint64, ctx), DeviceArray(nblocks + 1, np.uint64, ctx)
+    dst = DeviceArray(nblocks * batch.MAX_BLOCK32 + 64, np.uint8, ctx)
+    slots = DeviceArray(nblocks * batch.SLOT32, np.uint8, ctx)
+    nb_out = C.c_size_t()
+
+    def jobs():
+        check(lib.utl_make_jobs(ctx.handle, d_val.ptr, d_blk.ptr, d_fl.ptr, lens.size, in_off.ptr, in_meta.ptr),
+              "jobs")
+
+    def slot():
+        jobs()
+        check(lib.utl_encode_blocks(ctx.handle, 0, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr, meta.ptr),
+              "encode")
+        check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
+        check(lib.utl_compact(ctx.handle, 0, slots.ptr, boff.ptr, nblocks, dst.ptr), "compact")
+
+    def direct():
+        jobs()
+        check(lib.utl_encode_u32_direct(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, dst.ptr, dst.nbytes,
+                                        meta.ptr, boff.ptr, C.byref(nb_out)), "direct")
+        check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
+
+    out = {}
+    try:
+        for m, fn in (("slot", slot), ("direct", direct)):
+            fn()
+            ts = []
+            for _ in range(reps):
+                e0, e1 = Event(), Event()
+                e0.record(ctx)
+                fn()
+                e1.record(ctx)
+                ctx.sync()
+                ts.append(e0.elapsed_ms(e1))
+            out[m] = round(float(np.median(ts)), 4)
+    finally:
+        for a in (d_val, d_blk, d_fl, in_off, in_meta, meta, boff, ooff, dst, slots):
+            a.free()
+    return out
 
 
 def tune_summary(tune: dict) -> dict:
