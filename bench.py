@@ -166,6 +166,18 @@ def _copy_gbps(ctx, nbytes: int) -> float:
     return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
 
 
+def _stream_ceiling(ctx, src, read_bytes: int, dst, write_bytes: int) -> float:
+    """GB/s of utl_dev_stream reading read_bytes of src and writing
+    write_bytes of dst (the compressed and decoded bytes of a decode; median
+    of 5 runs of 5 launches)."""
+    from utlpfor_amd.device import dev_stream
+    rd = read_bytes // 16 * 16
+    wr = write_bytes // 16 * 16
+    dev_stream(ctx, dst, wr, src, rd)
+    ms = float(np.median([_events_time(ctx, lambda: dev_stream(ctx, dst, wr, src, rd), 5) for _ in range(5)]))
+    return round((rd + wr) / (ms * 1e-3) / 1e9, 1)
+
+
 def _copy_calibration(ctx) -> None:
     """A plain copy kernel of known size (256 MiB read + 256 MiB written),
     whose FETCH_SIZE/WRITE_SIZE scale the decode kernel's counters."""
@@ -249,6 +261,9 @@ def headline_uneven(ctx, args) -> dict:
     # compressed bytes + decoded bytes + meta words + wave index entries
     algo_bytes = b_in + d_out + 4 * nb + 16 * ((nb + 7) // 8)
     achieved = algo_bytes / (dec_ms * 1e-3) / 1e9
+    # the practical ceiling for this traffic: a plain streaming kernel that
+    # reads the compressed bytes and writes as many bytes as the decode
+    ceiling = _stream_ceiling(ctx, s.data, b_in, out, d_out)
     t = batch.get_tune(ctx)
     workload = {"workload": f"{args.streams} uneven uint32 streams, lengths logU[1,200000], flags U{{0, Delta, NoFOR, "
                             "NoPatch, Delta|NoFOR}}, values U[0,2^b) b~U{0..32} or random walks (steps "
@@ -260,7 +275,8 @@ def headline_uneven(ctx, args) -> dict:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _traffic("decode_uneven200k", nb) if args.streams == 200_000 else None,
             "kernel": _kernel_name(t, meta=True, persistent=bool(tune["persistent"])),
-            "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes}
+            "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes,
+            "stream_ceiling_GBps": ceiling, "frac_of_stream_ceiling": round(achieved / ceiling, 4)}
     res = _result(value, args.steps, args.warmup, ms_per_step, workload, roof)
     res["step_ms_events"] = {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4)}
@@ -360,13 +376,15 @@ def headline_delta(ctx, args, cpu: bool = True, extras: bool = True) -> dict:
     dec_ms = _events_time(ctx, lambda: batch.decode(ctx, s, out=out, status=status), max(args.steps, 10))
     algo_bytes = b_in + d_out + 16 * nb
     achieved = algo_bytes / (dec_ms * 1e-3) / 1e9
+    ceiling = _stream_ceiling(ctx, s.data, b_in, out, d_out)
     workload = {"workload": "delta postings: 64Mi uint32, gaps 1+Geom(1/8), 2% U[2^16,2^24), mod 2^32, flag Delta, "
                             "524288 blocks, whole-stream index scan + decode per step",
                 "values": n, "blocks": nb, "compressed_bytes": b_in, "bit_exact": exact, "seed": 2}
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _traffic("decode_delta64m", nb) if n == 64 << 20 else None,
-            "kernel": _kernel_name(batch.get_tune(ctx)), "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes}
+            "kernel": _kernel_name(batch.get_tune(ctx)), "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes,
+            "stream_ceiling_GBps": ceiling, "frac_of_stream_ceiling": round(achieved / ceiling, 4)}
     res = _result(value, args.steps, args.warmup, ms_per_step, workload, roof)
     res["step_ms_events"] = {"p50": round(float(np.percentile(step_ms, 50)), 4),
                              "p99": round(float(np.percentile(step_ms, 99)), 4)}

@@ -62,6 +62,10 @@ hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t 
 
 // Plain streaming copy (16-byte vectors when both pointers allow it).
 hipError_t utl_launch_copy(uint8_t* dst, const uint8_t* src, uint64_t n, hipStream_t stream);
+// Streaming read of read_bytes and write of write_bytes (16-byte chunks;
+// 16-byte aligned pointers), on `grid` workgroups.
+hipError_t utl_launch_stream_rw(uint8_t* dst, uint64_t write_bytes, const uint8_t* src, uint64_t read_bytes,
+                                int grid, hipStream_t stream);
 
 // Gathered index of a batch of block ranges of a resident store (utl_serve.hip).
 hipError_t utl_launch_gather_index(const uint64_t* boff, const uint64_t* ooff, uint64_t store_nblocks,

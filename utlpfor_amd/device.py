@@ -144,6 +144,11 @@ def dev_copy(ctx: Context, dst: DeviceArray, src: DeviceArray, nbytes: int) -> N
     check(_lib.load().utl_dev_copy(ctx.handle, dst.ptr, src.ptr, nbytes), "utl_dev_copy")
 
 
+def dev_stream(ctx: Context, dst: DeviceArray, write_bytes: int, src: DeviceArray, read_bytes: int) -> None:
+    """The streaming kernel of a given traffic shape (utl_dev_stream)."""
+    check(_lib.load().utl_dev_stream(ctx.handle, dst.ptr, write_bytes, src.ptr, read_bytes), "utl_dev_stream")
+
+
 class PinnedBuffer:
     """Page-locked host memory (utl_host_alloc): device copies into it run
     at the full link rate (on MI355X about 57 GB/s against 10-17 GB/s into
