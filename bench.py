@@ -266,7 +266,7 @@ def headline_uneven(ctx, args) -> dict:
     ceiling = _stream_ceiling(ctx, s.data, b_in, out, d_out)
     t = batch.get_tune(ctx)
     workload = {"workload": f"{args.streams} uneven uint32 streams, lengths logU[1,200000], flags U{{0, Delta, NoFOR, "
-                            "NoPatch, Delta|NoFOR}}, values U[0,2^b) b~U{0..32} or random walks (steps "
+                            "NoPatch, Delta|NoFOR}, values U[0,2^b) b~U{0..32} or random walks (steps "
                             "U[-2^10,2^10]); one hipGraph launch (device prefix scan of the block lengths + decode of "
                             "every block) per step",
                 "streams": args.streams, "values": n, "blocks": nb, "compressed_bytes": b_in, "bit_exact": exact,

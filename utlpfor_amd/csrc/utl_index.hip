@@ -199,10 +199,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_groups(uint64_t* __restri
 // before it. MODE 1: they hold prefixes within groups of kGroupTiles tiles
 // (k_scan_groups) and each workgroup sums the group totals before its own.
 // MODE 2: as 1, with the group totals already turned into exclusive group
-// prefixes (k_scan_tiles over the groups). WAVE: instead of every block's
-// offsets, write only those of blocks 8w (wave[2w], wave[2w+1]): the wave
-// index a decode in meta form reads.
-template <int MODE, bool WAVE = false>
+// prefixes (k_scan_tiles over the groups).
+template <int MODE>
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __restrict__ meta, uint64_t n,
                                                                const uint64_t* __restrict__ tile_len,
                                                                const uint64_t* __restrict__ tile_cnt,
@@ -250,10 +248,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __
     b[k + 1] = b[k] + l[k];
     o[k + 1] = o[k] + c[k];
   }
-  if (WAVE) {
-    if ((i0 & 7) == 0 && i0 < n) *reinterpret_cast<ulonglong2*>(boff + (i0 >> 2)) = make_ulonglong2(b[0], o[0]);
-    return;
-  }
   if (i0 + 4 <= n) {
     *reinterpret_cast<ulonglong2*>(boff + i0) = make_ulonglong2(b[0], b[1]);
     *reinterpret_cast<ulonglong2*>(boff + i0 + 2) = make_ulonglong2(b[2], b[3]);
@@ -273,6 +267,111 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply4(const uint32_t* __
       boff[n] = b[k + 1];
       ooff[n] = o[k + 1];
     }
+}
+
+// The wave index in tiles of kWaveTile blocks: thread t owns blocks 16t ..
+// 16t + 15 of its tile (four 16-byte meta loads in flight) and writes the
+// entries of its two groups of eight (32 contiguous bytes). Four times the
+// blocks per thread of the block-index kernels, for more bytes in flight per
+// wave and a quarter of the tiles; the passes and MODEs are theirs.
+constexpr int kWaveItems = 16;
+constexpr uint64_t kWaveTile = (uint64_t)kScanThreads * kWaveItems;
+
+__device__ __forceinline__ void load_meta16(const uint32_t* __restrict__ meta, uint64_t n, uint64_t i,
+                                            uint32_t* m) {
+#pragma unroll
+  for (int q = 0; q < kWaveItems / 4; ++q) {
+    const uint64_t k = i + 4 * q;
+    if (k + 4 <= n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(meta + k);
+      m[4 * q] = v.x;
+      m[4 * q + 1] = v.y;
+      m[4 * q + 2] = v.z;
+      m[4 * q + 3] = v.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) m[4 * q + e] = k + e < n ? meta[k + e] : 0u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_wave_reduce(const uint32_t* __restrict__ meta, uint64_t n,
+                                                               uint64_t* __restrict__ tile_len,
+                                                               uint64_t* __restrict__ tile_cnt) {
+  __shared__ uint64_t lds[2][4];
+  uint32_t m[kWaveItems];
+  load_meta16(meta, n, (uint64_t)blockIdx.x * kWaveTile + kWaveItems * threadIdx.x, m);
+  uint64_t sl = 0, sc = 0;
+#pragma unroll
+  for (int k = 0; k < kWaveItems; ++k) {
+    sl += m[k] & 0xFFFFFFu;
+    sc += m[k] >> 24;
+  }
+  uint64_t tl, tc;
+  wg_excl_scan(sl, lds[0], tl);
+  wg_excl_scan(sc, lds[1], tc);
+  if (threadIdx.x == 0) {
+    tile_len[blockIdx.x] = tl;
+    tile_cnt[blockIdx.x] = tc;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kScanThreads) void k_wave_apply(const uint32_t* __restrict__ meta, uint64_t n,
+                                                              const uint64_t* __restrict__ tile_len,
+                                                              const uint64_t* __restrict__ tile_cnt,
+                                                              const uint64_t* __restrict__ group_len,
+                                                              const uint64_t* __restrict__ group_cnt,
+                                                              uint64_t* __restrict__ wave) {
+  __shared__ uint64_t lds[4][4];
+  const uint64_t i0 = (uint64_t)blockIdx.x * kWaveTile + kWaveItems * threadIdx.x;
+  uint32_t m[kWaveItems];
+  load_meta16(meta, n, i0, m);
+  uint64_t tile_pl, tile_pc;
+  if (MODE == 0) {
+    uint64_t sl = 0, sc = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) {
+      sl += tile_len[t];
+      sc += tile_cnt[t];
+    }
+    wg_excl_scan(sl, lds[2], tile_pl);
+    wg_excl_scan(sc, lds[3], tile_pc);
+  } else if (MODE == 1) {
+    const uint32_t grp = (uint32_t)(blockIdx.x / kGroupTiles);
+    uint64_t sl = 0, sc = 0;
+    for (uint32_t t = threadIdx.x; t < grp; t += kScanThreads) {
+      sl += group_len[t];
+      sc += group_cnt[t];
+    }
+    wg_excl_scan(sl, lds[2], tile_pl);
+    wg_excl_scan(sc, lds[3], tile_pc);
+    tile_pl += tile_len[blockIdx.x];
+    tile_pc += tile_cnt[blockIdx.x];
+  } else {
+    const uint64_t grp = blockIdx.x / kGroupTiles;
+    tile_pl = tile_len[blockIdx.x] + group_len[grp];
+    tile_pc = tile_cnt[blockIdx.x] + group_cnt[grp];
+  }
+  uint64_t l8 = 0, c8 = 0, sl = 0, sc = 0;
+#pragma unroll
+  for (int k = 0; k < kWaveItems; ++k) {
+    if (k == 8) {
+      l8 = sl;
+      c8 = sc;
+    }
+    sl += m[k] & 0xFFFFFFu;
+    sc += m[k] >> 24;
+  }
+  uint64_t tl, tc;
+  const uint64_t pl = wg_excl_scan(sl, lds[0], tl) + tile_pl;
+  const uint64_t pc = wg_excl_scan(sc, lds[1], tc) + tile_pc;
+  uint64_t* w = wave + 2 * (i0 >> 3);
+  if (i0 + 8 < n) {
+    *reinterpret_cast<ulonglong2*>(w) = make_ulonglong2(pl, pc);
+    *reinterpret_cast<ulonglong2*>(w + 2) = make_ulonglong2(pl + l8, pc + c8);
+  } else if (i0 < n) {
+    *reinterpret_cast<ulonglong2*>(w) = make_ulonglong2(pl, pc);
+  }
 }
 
 // BlockLength of every block with a known start (header-parse pass).
@@ -393,23 +492,23 @@ size_t utl_wave_index_entries(uint64_t nblocks) { return (size_t)(2 * ((nblocks 
 hipError_t utl_launch_wave_index(const uint32_t* meta, uint64_t n, uint64_t* wave, void* tmp, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   if ((((uintptr_t)meta | (uintptr_t)wave) & 15) != 0) return hipErrorInvalidValue;
-  const uint64_t nt = (n + kTile4 - 1) / kTile4;
+  const uint64_t nt = (n + kWaveTile - 1) / kWaveTile;  // fewer than the kTile4 tiles tmp is sized for
   const uint64_t ng = (nt + kGroupTiles - 1) / kGroupTiles;
   uint64_t* tl = (uint64_t*)tmp;
   uint64_t* tc = tl + nt;
   uint64_t* gl = tc + nt;
   uint64_t* gc = gl + ng;
   const dim3 grid((unsigned)nt), block(kScanThreads);
-  k_scan_reduce4<<<grid, block, 0, stream>>>(meta, n, tl, tc);
+  k_wave_reduce<<<grid, block, 0, stream>>>(meta, n, tl, tc);
   if (nt <= kInlineTiles) {
-    k_scan_apply4<0, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+    k_wave_apply<0><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave);
   } else {
     k_scan_groups<<<dim3((unsigned)ng), block, 0, stream>>>(tl, tc, nt, gl, gc);
     if (ng <= kInlineTiles) {
-      k_scan_apply4<1, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+      k_wave_apply<1><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave);
     } else {
       k_scan_tiles<<<1, kScanThreads, 0, stream>>>(gl, gc, ng);
-      k_scan_apply4<2, true><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave, nullptr);
+      k_wave_apply<2><<<grid, block, 0, stream>>>(meta, n, tl, tc, gl, gc, wave);
     }
   }
   return hipGetLastError();
