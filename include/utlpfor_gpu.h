@@ -337,10 +337,12 @@ int utl_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Plain device-to-device copy kernel (bandwidth calibration). */
 int utl_dev_copy(utl_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 /* A streaming kernel that reads read_bytes from d_src and writes write_bytes
- * to d_dst (16-byte chunks, coalesced; 16-byte aligned pointers): the
- * practical HBM rate of a kernel with the same traffic (the decode of a
- * stream reads its compressed bytes and writes its values). */
-int utl_dev_stream(utl_ctx *ctx, void *d_dst, size_t write_bytes, const void *d_src, size_t read_bytes);
+ * to d_dst (16-byte chunks, coalesced; 16-byte aligned pointers) on
+ * grid_wgs workgroups of 256 threads (0: four per CU): the practical HBM
+ * rate of a kernel with the same traffic (the decode of a stream reads its
+ * compressed bytes and writes its values). */
+int utl_dev_stream(utl_ctx *ctx, void *d_dst, size_t write_bytes, const void *d_src, size_t read_bytes,
+                   int grid_wgs);
 
 /* ---- stream container (no reference counterpart) ----
  *

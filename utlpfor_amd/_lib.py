@@ -70,7 +70,7 @@ SIGNATURES = {
     "utl_event_record": (C.c_int, [vp, vp]),
     "utl_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
     "utl_dev_copy": (C.c_int, [vp, vp, vp, sz]),
-    "utl_dev_stream": (C.c_int, [vp, vp, sz, vp, sz]),
+    "utl_dev_stream": (C.c_int, [vp, vp, sz, vp, sz, C.c_int]),
     "utl_container_bytes": (sz, [sz, sz, sz]),
     "utl_container_write": (C.c_int, [vp, sz, vp, sz, vp, sz, vp, sz, C.POINTER(sz)]),
     "utl_container_parse": (C.c_int, [vp, sz, vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),

@@ -834,8 +834,9 @@ int utl_dev_copy(utl_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
   return UTL_OK;
 }
 
-int utl_dev_stream(utl_ctx* ctx, void* d_dst, size_t write_bytes, const void* d_src, size_t read_bytes) {
-  if ((write_bytes && !d_dst) || (read_bytes && !d_src)) return UTL_ERR_ARGUMENT;
+int utl_dev_stream(utl_ctx* ctx, void* d_dst, size_t write_bytes, const void* d_src, size_t read_bytes,
+                   int grid_wgs) {
+  if ((write_bytes && !d_dst) || (read_bytes && !d_src) || grid_wgs < 0) return UTL_ERR_ARGUMENT;
   if ((((uintptr_t)d_dst | (uintptr_t)d_src) & 15) != 0) {
     g_last_error = "utl_dev_stream: pointers must be 16-byte aligned";
     return UTL_ERR_ARGUMENT;
@@ -843,8 +844,8 @@ int utl_dev_stream(utl_ctx* ctx, void* d_dst, size_t write_bytes, const void* d_
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_stream_rw((uint8_t*)d_dst, write_bytes, (const uint8_t*)d_src, read_bytes, 4 * c->cu_count,
-                               c->stream));
+  HIP_TRY(utl_launch_stream_rw((uint8_t*)d_dst, write_bytes, (const uint8_t*)d_src, read_bytes,
+                               grid_wgs ? grid_wgs : 4 * c->cu_count, c->stream));
   return UTL_OK;
 }
 

@@ -144,9 +144,11 @@ def dev_copy(ctx: Context, dst: DeviceArray, src: DeviceArray, nbytes: int) -> N
     check(_lib.load().utl_dev_copy(ctx.handle, dst.ptr, src.ptr, nbytes), "utl_dev_copy")
 
 
-def dev_stream(ctx: Context, dst: DeviceArray, write_bytes: int, src: DeviceArray, read_bytes: int) -> None:
+def dev_stream(ctx: Context, dst: DeviceArray, write_bytes: int, src: DeviceArray, read_bytes: int,
+               grid_wgs: int = 0) -> None:
     """The streaming kernel of a given traffic shape (utl_dev_stream)."""
-    check(_lib.load().utl_dev_stream(ctx.handle, dst.ptr, write_bytes, src.ptr, read_bytes), "utl_dev_stream")
+    check(_lib.load().utl_dev_stream(ctx.handle, dst.ptr, write_bytes, src.ptr, read_bytes, grid_wgs),
+          "utl_dev_stream")
 
 
 class PinnedBuffer:
