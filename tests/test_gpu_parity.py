@@ -389,6 +389,42 @@ def test_uneven_streams_graph_autotune(gpu, orc):
     s.free()
 
 
+def test_graph_replays_after_context_scratch_grows(gpu, orc):
+    """A decode graph owns its scan scratch: replays stay exact after the
+    context's shared scratch is freed and reallocated by a much larger index
+    scan on the same context (both graph forms)."""
+    from utlpfor_amd import _lib, batch, workloads
+    from utlpfor_amd.device import DeviceArray, check
+    codec, ctx = gpu
+    lib = _lib.load()
+    v, lens, flags = workloads.uneven_streams(40, max_len=5000, seed=12)
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    out = DeviceArray(v.size, np.uint32, ctx)
+    st = DeviceArray(s.nblocks, np.int32, ctx)
+    rng = np.random.default_rng(4)
+    nmax = 12 << 20
+    big = DeviceArray.from_numpy((rng.integers(4, 1018, nmax) | (128 << 24)).astype(np.uint32), ctx)
+    bo, oo = DeviceArray(nmax + 1, np.uint64, ctx), DeviceArray(nmax + 1, np.uint64, ctx)
+    for block_index, nbig in ((True, 6 << 20), (False, nmax)):
+        # each scan needs more scratch than the context held when the graph was captured
+        say(f"graph with block_index={block_index}, {s.nblocks} blocks, then a {nbig}-block index scan")
+        g = batch.DecodeGraph(ctx, s, out, st, persistent=True, block_index=block_index)
+        check(lib.utl_index_build(ctx.handle, big.ptr, nbig, bo.ptr, oo.ptr), "index")
+        ctx.sync()
+        for rep in range(3):
+            out.fill_bytes(0xA5)
+            ctx.sync()
+            g.launch()
+            ctx.sync()
+            assert np.all(st.to_numpy() == 0) and np.array_equal(out.to_numpy(), v), (block_index, rep)
+            check(lib.utl_index_build(ctx.handle, big.ptr, nbig, bo.ptr, oo.ptr), "index")
+            ctx.sync()
+        g.close()
+    for a in (big, bo, oo, out, st):
+        a.free()
+    s.free()
+
+
 def test_batched_u64(gpu, orc):
     from utlpfor_amd import batch
     from utlpfor_amd.device import DeviceArray
