@@ -245,6 +245,10 @@ def tune_summary(tune: dict) -> dict:
         k = f"{c['arena_bytes']}B/{c['waves_per_wg']}w/{'persistent' if c['persistent'] else 'flat'}"
         best[k] = round(min(best.get(k, 1e30), c["ms"]), 5)
     rec = {k: tune[k] for k in ("arena_bytes", "waves_per_wg", "grid_wgs", "tiles_per_grab", "persistent")}
+    # the shapes these knobs give: 8 blocks per wave at a time (8 threads a
+    # block), blocks per workgroup tile, and blocks a wave takes per grab
+    rec.update(blocks_per_wave_round=8, blocks_per_tile=8 * tune["waves_per_wg"],
+               blocks_per_wave_grab=8 * tune["tiles_per_grab"] if tune["persistent"] else 8)
     rec.update(ms=round(tune["ms"], 5), variants_ms=best)
     return rec
 
