@@ -569,9 +569,10 @@ __device__ __forceinline__ void store_u32(uint32_t* __restrict__ out, uint64_t o
 // The values of one round whose blocks fill the output range [g0, g1)
 // without holes (every block decoded, each filling its whole slot), staged
 // in aux in output order and written out by the wave as 16-byte pieces of
-// the range: non-temporal where a piece's 64-byte segment lies inside the
-// range, cached at its two ends. Blocks at odd offsets or shorter than 128
-// values (streams of arbitrary lengths) then still write whole segments.
+// the range: every piece inside the range non-temporal, the values of the
+// partial pieces at its two ends as cached 4-byte stores. Blocks at odd
+// offsets or shorter than 128 values (streams of arbitrary lengths) then
+// still write whole 16-byte pieces.
 // aux is free: every exception read of the round is done. act: the lane's
 // block is in the round and has count values at out0.
 template <class WL>
@@ -602,18 +603,16 @@ __device__ __forceinline__ void store_u32_staged(WL* W, uint32_t* __restrict__ o
   wave_fence();
   // piece k: values [4k - sh, 4k - sh + 4) of the range, staged at y[4k]
   uint32_t* const ob = out + g0 - sh;  // 16-byte aligned
-  const int seg0 = (int)((reinterpret_cast<uintptr_t>(ob) >> 4) & 3);  // piece 0's place in its segment
   const int npieces = (sh + n + 3) >> 2;  // <= 257: y[4k + 3] < kAux
 #pragma unroll 1
   for (int k = lane_id(); k < npieces; k += 64) {
     const uint4 q = *reinterpret_cast<const uint4*>(&y[4 * k]);
     const int v = 4 * k - sh;  // first value of the piece, relative to g0
-    const int pos = (seg0 + k) & 3;  // the piece's place in its 64-byte segment
-    const int s = v - 4 * pos;       // first value of the segment
     uint4* o = reinterpret_cast<uint4*>(ob + 4 * k);
-    // non-temporal when the whole segment is in the range and written by
-    // this one store instruction (its four pieces on lanes of one pass)
-    if (s >= 0 && s + 16 <= n && ((k - pos) & 63) <= 60) {
+    // every whole piece non-temporal (measured on 200,000 uneven streams:
+    // 4.03 ms, against 4.06 when only the pieces of segments wholly inside
+    // the range and in one store instruction were, and 4.46 with none)
+    if (v >= 0 && v + 4 <= n) {
       typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
       u32x4 t;
       t.x = q.x;
@@ -621,8 +620,6 @@ __device__ __forceinline__ void store_u32_staged(WL* W, uint32_t* __restrict__ o
       t.z = q.z;
       t.w = q.w;
       __builtin_nontemporal_store(t, reinterpret_cast<u32x4*>(o));
-    } else if (v >= 0 && v + 4 <= n) {
-      *o = q;
     } else {
       uint32_t* e = ob + 4 * k;
       if (v >= 0 && v < n) e[0] = q.x;
