@@ -39,12 +39,11 @@ __device__ __forceinline__ void lds_fence() {
 // registers and the payload words of its two lanes are packed from
 // registers.
 //
-// Slot layout: a uint32 block starts at slot byte sh = (-poff) & 3, which
-// puts its payload (and everything after it) on a 4-byte boundary, so all
-// stores are aligned dwords; the bytes before it are 0. A uint64 block
-// starts at slot byte 0 (sh = 0) and is written with the unaligned writers
-// below, as its second sub-block starts at any byte. A block's first byte
-// (its count) is never 0, which is how the compaction finds sh.
+// Slot layout. uint32: the 32 blocks of a workgroup tile back to back from
+// the tile's first slot (offsets from a wave scan of the block lengths and
+// the workgroup's wave totals), so the compaction copies each tile's bytes
+// in one piece. uint64: block b from byte 0 of slot b. Both are written
+// with the writers below, which take any byte address.
 
 constexpr int kEncGrpWaves = 4;
 constexpr int kWaveBlocksEnc = 8;  // blocks per wavefront
@@ -266,45 +265,7 @@ __device__ __forceinline__ void tail_write(const EncGrpLds* L, int blk, int j, i
   }
 }
 
-// UTL pack of lanes 2j, 2j+1 (scalar.go:7-59): lane L's word w goes to
-// payload dword 16w + L. v0/v1 hold values already masked to BW bits.
-template <int BW>
-__device__ __forceinline__ void pack_bw(const uint32_t* v0, const uint32_t* v1, uint32_t* __restrict__ dst, int j) {
-  if constexpr (BW > 0) {
-    constexpr int NW = BW / 4;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      uint32_t a = 0, c = 0;
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        const int s = r * BW - 32 * w;
-        if (s > -BW && s < 32) {
-          a |= s >= 0 ? (v0[r] << s) : (v0[r] >> (-s));
-          c |= s >= 0 ? (v1[r] << s) : (v1[r] >> (-s));
-        }
-      }
-      dst[16 * w + 2 * j] = a;
-      dst[16 * w + 2 * j + 1] = c;
-    }
-  }
-}
-
-__device__ __forceinline__ void pack_any(int bw, const uint32_t* v0, const uint32_t* v1, uint32_t* __restrict__ dst,
-                                         int j) {
-  switch (bw) {
-    case 4: pack_bw<4>(v0, v1, dst, j); break;
-    case 8: pack_bw<8>(v0, v1, dst, j); break;
-    case 12: pack_bw<12>(v0, v1, dst, j); break;
-    case 16: pack_bw<16>(v0, v1, dst, j); break;
-    case 20: pack_bw<20>(v0, v1, dst, j); break;
-    case 24: pack_bw<24>(v0, v1, dst, j); break;
-    case 28: pack_bw<28>(v0, v1, dst, j); break;
-    case 32: pack_bw<32>(v0, v1, dst, j); break;
-    default: break;
-  }
-}
-
-// ---- writing a block at an arbitrary byte offset (the direct form) ----
+// ---- writing a block at an arbitrary byte offset ----
 //
 // A block of the concatenated stream starts at any byte. Its bytes are
 // written in three parts, each byte exactly once and no byte outside the
@@ -877,17 +838,16 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u64(const uint64_t
   if (j == 0 && mine) out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
 }
 
-// Concatenates the slots into dst (block b's bytes at dst + boff[b]). Each
-// block writes the aligned dwords inside its byte range with dword stores
-// and its bytes of the partial dwords at its two ends with byte stores (the
-// neighbouring blocks write the other bytes of those), so it reads only its
-// own slot bytes and nothing past the stream is written. Eight lanes per
-// block, eight blocks per wavefront: the index entries and the slot's first
-// dword (where the block starts, after the encoders' zero bytes) are loaded
-// together, then every slot dword the block needs (up to 64 per round, eight
-// per lane, each as two dword loads aligned in registers). (The previous
-// form also read the next slot's first bytes and whole rounds past the
-// block's end: 2.2x the stream in reads, 98.6 us for the 64 Mi delta stream.)
+// Concatenates slots of one block each (the uint64 layout) into dst (block
+// b's bytes at dst + boff[b]). Each block writes the aligned dwords inside
+// its byte range with dword stores and its bytes of the partial dwords at
+// its two ends with byte stores (the neighbouring blocks write the other
+// bytes of those), so it reads only its own slot bytes and nothing past the
+// stream is written. Eight lanes per block, eight blocks per wavefront: the
+// index entries and the slot's first dword (the block starts after any
+// zero bytes an encoder put before it; its first byte, the count, is never
+// 0) are loaded together, then every slot dword the block needs (up to 64
+// per round, eight per lane, each as two dword loads aligned in registers).
 constexpr int kCompactPer = 8;  // output dwords per lane per round
 
 __global__ __launch_bounds__(256) void k_compact(const uint8_t* __restrict__ slots, uint32_t slot_bytes,
