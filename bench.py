@@ -144,15 +144,18 @@ def _timed_steps(ctx, step, steps: int):
     return wall * 1e3 / steps, [ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]
 
 
-def _traffic(name: str, nblocks: int):
+def _traffic(name: str, nblocks: int, kernel: str | None = None):
     """HBM bytes per decode dispatch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes over this workload (copy-kernel calibrated)."""
+    WRITE_SIZE passes over this workload (copy-kernel calibrated), when they
+    were taken of the same kernel variant."""
     tpath = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(tpath):
         return None
     with open(tpath) as f:
         t = json.load(f)
-    return t.get("traffic_bytes") if t.get("blocks") == nblocks else None
+    if t.get("blocks") != nblocks or (kernel and not t.get("kernel", "").startswith(kernel.rstrip(">"))):
+        return None
+    return t.get("traffic_bytes")
 
 
 def _copy_gbps(ctx, nbytes: int) -> float:
@@ -233,18 +236,40 @@ def headline_uneven(ctx, args) -> dict:
     log(f"{nb} blocks, {b_in} compressed bytes ({b_in / n:.3f} B/value)")
     out = DeviceArray(n, np.uint32, ctx)
     status = DeviceArray(nb, np.int32, ctx)
-    if args.tune:
-        tune = _parse_tune(args.tune, cus)
-        batch.set_tune(ctx, tune["arena_bytes"], tune["waves_per_wg"], tune["grid_wgs"], tune["tiles_per_grab"])
-        tsum = dict(tune, fixed=True)
-    else:
-        log("autotuning the decode launch")
-        tune = autotune.tune_decode(ctx, s, out, status, use_cache=False, reps=3, meta_form=True)
-        tsum = runbench.tune_summary(tune)
-    log(f"launch variant: {tsum}")
-    # the graph: wave index (the device prefix scan of block lengths, one
-    # entry per eight blocks) + decode from the meta words
-    graph = batch.DecodeGraph(ctx, s, out, status, persistent=bool(tune["persistent"]), block_index=False)
+    # the two graph forms: the wave index (one (byte, value) offset pair per
+    # eight blocks) + decode from the meta words, or the block index
+    # (block_off / out_off) + decode from it; each in the launch variant
+    # tuned for it
+    forms_wanted = ("meta", "block") if args.form == "auto" else (args.form,)
+    graphs, tunes = {}, {}
+    for form in forms_wanted:
+        meta_form = form == "meta"
+        if args.tune:
+            tune = _parse_tune(args.tune, cus)
+            batch.set_tune(ctx, tune["arena_bytes"], tune["waves_per_wg"], tune["grid_wgs"], tune["tiles_per_grab"])
+            tsum = dict(tune, fixed=True)
+        else:
+            log(f"autotuning the decode launch ({form} form)")
+            tune = autotune.tune_decode(ctx, s, out, status, use_cache=False, reps=3, meta_form=meta_form)
+            tsum = runbench.tune_summary(tune)
+        log(f"{form} form launch variant: {tsum}")
+        tunes[form] = (tune, tsum, batch.get_tune(ctx))
+        graphs[form] = batch.DecodeGraph(ctx, s, out, status, persistent=bool(tune["persistent"]),
+                                         block_index=not meta_form)
+    # the faster graph (interleaved short runs of each), then the timed steps on it
+    step_trial = {}
+    for _ in range(3):
+        for form, g in graphs.items():
+            g.launch()
+            ctx.sync()
+            step_trial.setdefault(form, []).append(_events_time(ctx, g.launch, 5))
+    step_trial = {f: float(np.median(t)) for f, t in step_trial.items()}
+    form = min(step_trial, key=step_trial.get)
+    tune, tsum, installed = tunes[form]
+    batch.set_tune(ctx, installed["arena_bytes"], installed["waves_per_wg"], installed["grid_wgs"],
+                   installed["tiles_per_grab"])
+    graph = graphs[form]
+    log(f"graph form: {form} (trial steps, ms: {step_trial})")
     wave = batch.build_wave_index(ctx, s)
     for _ in range(args.warmup):
         graph.launch()
@@ -262,26 +287,32 @@ def headline_uneven(ctx, args) -> dict:
     ms_per_step, step_ms = _timed_steps(ctx, graph.launch, args.steps)
     value = d_out / (ms_per_step * 1e-3) / 1e9
     # dominant kernel alone: the graph's decode launch between two events
-    dec_ms = _events_time(ctx, lambda: batch.decode_meta(ctx, s, wave, out=out, status=status,
-                                                         persistent=bool(tune["persistent"])), max(args.steps, 10))
-    # compressed bytes + decoded bytes + meta words + wave index entries
-    algo_bytes = b_in + d_out + 4 * nb + 16 * ((nb + 7) // 8)
+    persistent = bool(tune["persistent"])
+    if form == "meta":
+        decode_fn = lambda: batch.decode_meta(ctx, s, wave, out=out, status=status, persistent=persistent)  # noqa: E731
+        # compressed bytes + decoded bytes + meta words + wave index entries
+        algo_bytes = b_in + d_out + 4 * nb + 16 * ((nb + 7) // 8)
+    else:
+        decode_fn = lambda: batch.decode(ctx, s, out=out, status=status, persistent=persistent)  # noqa: E731
+        # compressed bytes + decoded bytes + block index entries
+        algo_bytes = b_in + d_out + 16 * nb
+    dec_ms = _events_time(ctx, decode_fn, max(args.steps, 10))
     achieved = algo_bytes / (dec_ms * 1e-3) / 1e9
     # the practical ceiling for this traffic: a plain streaming kernel that
     # reads the compressed bytes and writes as many bytes as the decode
     ceiling = _stream_ceiling(ctx, s.data, b_in, out, d_out)
     t = batch.get_tune(ctx)
+    kname = _kernel_name(t, meta=form == "meta", persistent=persistent)
     workload = {"workload": f"{args.streams} uneven uint32 streams, lengths logU[1,200000], flags U{{0, Delta, NoFOR, "
                             "NoPatch, Delta|NoFOR}, values U[0,2^b) b~U{0..32} or random walks (steps "
                             "U[-2^10,2^10]); one hipGraph launch (device prefix scan of the block lengths + decode of "
                             "every block) per step",
                 "streams": args.streams, "values": n, "blocks": nb, "compressed_bytes": b_in, "bit_exact": exact,
-                "seed": 3, "gen_seconds": round(gen_s, 1)}
+                "seed": 3, "gen_seconds": round(gen_s, 1), "graph_form": form}
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic("decode_uneven200k", nb) if args.streams == 200_000 else None,
-            "kernel": _kernel_name(t, meta=True, persistent=bool(tune["persistent"])),
-            "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes,
+            "traffic": _traffic(f"decode_uneven200k_{form}", nb, kname) if args.streams == 200_000 else None,
+            "kernel": kname, "kernel_ms": round(dec_ms, 5), "algo_bytes": algo_bytes,
             "stream_ceiling_GBps": ceiling, "frac_of_stream_ceiling": round(achieved / ceiling, 4)}
     res = _result(value, args.steps, args.warmup, ms_per_step, workload, roof)
     res["step_ms_events"] = {"p50": round(float(np.percentile(step_ms, 50)), 4),
@@ -289,7 +320,10 @@ def headline_uneven(ctx, args) -> dict:
     if args.copy_calibration:
         _copy_calibration(ctx)
     forms = {"graph_step_ms": round(ms_per_step, 4), "decode_kernel_ms": round(dec_ms, 5), "tune": tsum,
-             "graph": "wave index (2 passes) + decode in meta form"}
+             "graph": ("wave index (2 passes) + decode in meta form" if form == "meta" else
+                       "block index scan + decode from the block index"),
+             "graph_trial_step_ms": {f: round(x, 5) for f, x in step_trial.items()},
+             "tune_other_form": {f: tunes[f][1] for f in tunes if f != form}}
     if not args.no_extras:
         forms["persistent_decode_ms"] = round(_events_time(
             ctx, lambda: batch.decode_meta(ctx, s, wave, out=out, status=status, persistent=True), 5), 5)
@@ -322,7 +356,8 @@ def headline_uneven(ctx, args) -> dict:
             log(f"cpu baseline (oracle) on {k} blocks")
             res["cpu_baseline"] = cpu_baseline(sb, boff_h[:k + 1], ooff_h, int(ooff_h[-1]),
                                                "the first 1/32 of the stream set's blocks")
-    graph.close()
+    for g in graphs.values():
+        g.close()
     for a in (wave, out, status):
         a.free()
     s.free()
@@ -432,6 +467,9 @@ def main() -> int:
     ap.add_argument("--values", type=int, default=64 << 20, help="values of the delta64m / walk workloads")
     ap.add_argument("--tune", default="", help="fixed decode variant ARENA:WAVES (flat) or ARENA:WAVES:GRID:GRAB "
                                                "(persistent); skips the autotuner")
+    ap.add_argument("--form", default="auto", choices=["auto", "meta", "block"],
+                    help="uneven workload: graph form (wave index + meta-form decode, or block index + decode); "
+                         "auto times both and keeps the faster")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the headline step (for profiling)")
     ap.add_argument("--store-gb", type=float, default=16.0, help="serve: compressed store size")

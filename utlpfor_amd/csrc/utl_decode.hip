@@ -709,13 +709,17 @@ __device__ __forceinline__ void decode_group(WL* W, const uint8_t* __restrict__ 
   // this lane's block: bytes [my_o0, my_o1) of src, slot_n values at out0
   uint64_t my_o0, my_o1, out0, slot_n;
   if (META) {
-    const uint32_t m = mine ? ix.meta[b] : 0u;
-    const uint32_t ln = m & 0xFFFFFFu, cn = m >> 24;
-    const uint32_t il = wave_group_scan(ln), ic = wave_group_scan(cn);  // inclusive over the 8 blocks
+    // the group's wave-index entry (uniform: scalar loads) and the lanes'
+    // meta words, both loads in flight at once (no branch between them: a
+    // lane past the last block reads block b0's word and drops it)
     const uint64_t w = b0 >> 3;
-    my_o0 = ix.wave[2 * w] + (il - ln);
+    const uint64_t wb = ix.wave[2 * w], wv = ix.wave[2 * w + 1];
+    const uint32_t m = ix.meta[b];
+    const uint32_t ln = mine ? m & 0xFFFFFFu : 0u, cn = mine ? m >> 24 : 0u;
+    const uint32_t il = wave_group_scan(ln), ic = wave_group_scan(cn);  // inclusive over the 8 blocks
+    my_o0 = wb + (il - ln);
     my_o1 = my_o0 + ln;
-    out0 = ix.wave[2 * w + 1] + (ic - cn);
+    out0 = wv + (ic - cn);
     slot_n = cn;
   } else {
     my_o0 = ix.boff[b];
