@@ -325,21 +325,18 @@ def headline_uneven(ctx, args) -> dict:
              "graph_trial_step_ms": {f: round(x, 5) for f, x in step_trial.items()},
              "tune_other_form": {f: tunes[f][1] for f in tunes if f != form}}
     if not args.no_extras:
-        forms["persistent_decode_ms"] = round(_events_time(
-            ctx, lambda: batch.decode_meta(ctx, s, wave, out=out, status=status, persistent=True), 5), 5)
-        forms["persistent_ok"] = bool(np.all(status.to_numpy() == 0))
-        forms["flat_decode_ms"] = round(
-            _events_time(ctx, lambda: batch.decode_meta(ctx, s, wave, out=out, status=status), 5), 5)
-        # the block form: full index scan + decode from block_off / out_off
-        lib_ = _lib.load()
-
-        def block_step():
-            check(lib_.utl_index_build(ctx.handle, s.meta.ptr, nb, s.block_off.ptr, s.out_off.ptr), "index")
-            batch.decode(ctx, s, out=out, status=status)
-        forms["block_index_step_ms"] = round(_events_time(ctx, block_step, 5), 5)
-        forms["block_form_decode_ms"] = round(
-            _events_time(ctx, lambda: batch.decode(ctx, s, out=out, status=status), 5), 5)
-        forms["block_form_ok"] = bool(np.all(status.to_numpy() == 0))
+        # each graph form's decode kernel alone, in the variant tuned for it
+        for f, (tn, _, inst) in tunes.items():
+            batch.set_tune(ctx, inst["arena_bytes"], inst["waves_per_wg"], inst["grid_wgs"], inst["tiles_per_grab"])
+            p = bool(tn["persistent"])
+            if f == "meta":
+                fn = lambda: batch.decode_meta(ctx, s, wave, out=out, status=status, persistent=p)  # noqa: E731
+            else:
+                fn = lambda: batch.decode(ctx, s, out=out, status=status, persistent=p)  # noqa: E731
+            forms[f"{f}_form_decode_ms"] = round(_events_time(ctx, fn, 5), 5)
+            forms[f"{f}_form_ok"] = bool(np.all(status.to_numpy() == 0))
+        batch.set_tune(ctx, installed["arena_bytes"], installed["waves_per_wg"], installed["grid_wgs"],
+                       installed["tiles_per_grab"])
         forms["copy_kernel_GBps"] = _copy_gbps(ctx, 4 << 30)
         boff_h = s.block_off.to_numpy()
         rng = np.random.default_rng(0)
