@@ -7,11 +7,12 @@ NoFOR, NoPatch, Delta|NoFOR}, values uniform in [0, 2^b) (b uniform 0..32) or
 signed random walks with steps U[-2^10, 2^10], packed by the GPU encoder
 into 25.7 M concatenated 128-value blocks (7.8 GB compressed, 13.1 GB
 decoded) resident in HBM. One step = one launch of a HIP graph holding the
-device prefix scan of the per-block lengths (the byte and value offsets of
-every eighth block: the wave index) and one decode launch of every block of
-every stream (each wavefront scans its eight blocks' lengths itself), in the
-launch variant the autotuner picks on the same stream. value = decoded GB/s
-of the whole step.
+device prefix scan of the per-block lengths and one decode launch of every
+block of every stream, in the faster of two graph forms on the same stream:
+the wave index (the byte and value offsets of every eighth block; each
+wavefront scans its eight blocks' lengths itself) + the meta-form decode,
+or the block index + the block-form decode, each in the launch variant the
+autotuner picks for it. value = decoded GB/s of the whole step.
 
 Also reported: the decode kernel's roofline from HIP events on its own
 stream, every launch variant the tuner timed, the persistent work-queue
