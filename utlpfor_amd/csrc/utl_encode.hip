@@ -109,19 +109,30 @@ __device__ __forceinline__ void choose_best_g(uint32_t lo, uint32_t hi, int max_
 // (ok0/ok1 false) hold 0 on entry and keep 0.
 
 // FOR decision decideFORFull (for.go:95-118), given the min mn and max mx of
-// the valid values: when FOR pays, subtracts the base from the valid values
-// and returns the FOR width code (1, 2, 3: a 1, 2 or 4 byte base); 0 = no FOR.
+// the valid values (n of them): when FOR pays, subtracts the base from the
+// valid values and returns the FOR width code (1, 2, 3: a 1, 2 or 4 byte
+// base); 0 = no FOR.
+// The standard width stdWidth needs the step histogram, except for a full
+// block whose values all share one step index s (step_len(mn) ==
+// step_len(mx)): every narrower candidate 4j then has all 128 values as
+// exceptions and costs 2 + 16 + 32 + 128 * ((k + 1) >> 1) - 64 k >= 50 bytes
+// more than width 4s (k = s - j >= 1 steps), so stdWidth = 4s.
 __device__ __forceinline__ int enc_for(uint32_t* x0, uint32_t* x1, const bool* ok0, const bool* ok1, uint32_t mn,
-                                       uint32_t mx, uint32_t& base) {
+                                       uint32_t mx, int n, uint32_t& base) {
   base = 0;
   if (mn == 0) return 0;
   const int for_max = ((bit_len32(mx - mn) + 3) >> 2) << 2;
   const int raw_max = ((bit_len32(mx) + 3) >> 2) << 2;
   if (for_max >= raw_max) return 0;
-  uint32_t lo, hi;
-  step_counts(x0, x1, lo, hi);
-  int std_w, unused;
-  choose_best_g(lo, hi, (int)step_len(mx), std_w, unused);
+  int std_w;
+  if (n == kBlock && step_len(mn) == step_len(mx)) {
+    std_w = raw_max;
+  } else {
+    uint32_t lo, hi;
+    step_counts(x0, x1, lo, hi);
+    int unused;
+    choose_best_g(lo, hi, (int)step_len(mx), std_w, unused);
+  }
   if (for_max >= std_w) return 0;
   base = mn;
 #pragma unroll
@@ -506,7 +517,7 @@ __device__ __forceinline__ void encode_u32_wave(EncGrpLds* L, uint64_t* wg, uint
   }
   uint32_t hflags = kTypeU32 | ((flag & UTL_SPECIAL) << 14);
   uint32_t base = 0;
-  const int fw = (flag & UTL_NOFOR) ? 0 : enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), base);
+  const int fw = (flag & UTL_NOFOR) ? 0 : enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), n, base);
   hflags |= (uint32_t)fw << 15;
   if (flag & UTL_DELTA) hflags |= enc_delta(x0, x1, ok0, ok1);
   int bw, exc;
@@ -729,7 +740,7 @@ __device__ __forceinline__ SubHead enc_sub_u64(EncGrpLds* L, int blk, int j, uin
         mx = max(mx, x1[r]);
       }
     }
-    fw = enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), base);
+    fw = enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), n, base);
   }
   hflags |= (for64 ? 3u : (uint32_t)fw) << 15;
   if (flag & UTL_DELTA) hflags |= enc_delta(x0, x1, ok0, ok1);
