@@ -284,6 +284,11 @@ def headline_uneven(ctx, args) -> dict:
         check(lib.utl_memcpy_d2h(ctx.handle, got.ctypes.data, out.ptr + 4 * a, 4 * m), "d2h")
         exact &= bool(np.array_equal(got, v[a:a + m]))
     log(f"round trip bit-exact: {exact}")
+    # random access over the whole 7.8 GB stream: 10 M uniformly random value
+    # positions, their expected values taken before the host copy goes
+    grng = np.random.default_rng(11)
+    get_pos = grng.integers(0, n, 10_000_000).astype(np.int64)
+    get_want = v[get_pos]
     del v
     ms_per_step, step_ms = _timed_steps(ctx, graph.launch, args.steps)
     value = d_out / (ms_per_step * 1e-3) / 1e9
@@ -345,6 +350,8 @@ def headline_uneven(ctx, args) -> dict:
         blobs = [s.data.read(int(boff_h[k]), int(boff_h[k + 1] - boff_h[k])).tobytes() for k in ks]
         log("per-call form")
         forms["per_call"] = per_call_form(ctx, blobs)
+        log("batched Get over the whole stream")
+        forms["get_large_store"] = get_large_store(ctx, s, get_pos, get_want)
         if not args.no_cpu:
             # a bounded sample: the first 1/32 of the blocks (streams are in
             # random order, so the sample has the stream mix of the whole)
@@ -369,6 +376,27 @@ def headline_uneven(ctx, args) -> dict:
         forms["served"] = served_form(ctx)
     res["forms"] = forms
     return res
+
+
+def get_large_store(ctx, s, pos, want) -> dict:
+    """One batched GetUint32 launch of `pos` (stream value positions, each
+    mapped to its block through the block index) over the resident stream,
+    timed with HIP events (median of 3 runs of 5 launches), its values and
+    statuses compared with `want`."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    dpos = DeviceArray.from_numpy(pos, ctx)
+    gout = DeviceArray(pos.size, np.uint32, ctx)
+    gst = DeviceArray(pos.size, np.int32, ctx)
+    fn = lambda: batch.get_batch(ctx, s, dpos, out=gout, status=gst)  # noqa: E731
+    fn()
+    ms = float(np.median([_events_time(ctx, fn, 5) for _ in range(3)]))
+    ok = bool((gst.to_numpy() == 0).all() and np.array_equal(gout.to_numpy(), want))
+    for a in (dpos, gout, gst):
+        a.free()
+    return {"positions": int(pos.size), "store_bytes": int(s.nbytes), "blocks": int(s.nblocks),
+            "kernel_ms": round(ms, 4), "Gget_per_s": round(pos.size / (ms * 1e-3) / 1e9, 3),
+            "ns_per_get": round(ms * 1e6 / pos.size, 4), "bit_exact": ok}
 
 
 def served_form(ctx) -> dict:

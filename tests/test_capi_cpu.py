@@ -86,6 +86,20 @@ def test_walk_rejects_bad_arguments_before_device_work(lib):
     assert lib.utl_index_walk(None, fake, fake, 1 << 32, None, None, None, fake, fake) == err
 
 
+def test_get_dir_rejects_bad_arguments_before_device_work(lib):
+    """utl_pos_dir_build and utl_get_batch_dir check their pointers before
+    they touch a device (UTL_ERR_ARGUMENT); the fake pointers are never
+    dereferenced."""
+    fake = 0x100000
+    err = 100  # UTL_ERR_ARGUMENT
+    assert lib.utl_pos_dir_build(None, None, 4, fake, 1) == err
+    assert lib.utl_pos_dir_build(None, fake, 4, None, 1) == err
+    assert lib.utl_pos_dir_build(None, fake, 0, fake, 1) == err
+    assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, None, 1, fake, 1, fake, fake) == err
+    assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, fake, 0, fake, 1, fake, fake) == err
+    assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, fake, 1, None, 1, fake, fake) == err
+
+
 def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
     from utlpfor_amd import _lib
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))

@@ -1,0 +1,141 @@
+"""Batched Get by stream position through the position directory
+(utl_pos_dir_build + utl_get_batch_dir) against the directory-less search
+of utl_get_batch and against the values: many short streams (a partial
+block at the end of each, so block numbers drift far from position / 128),
+boundary and out-of-range positions, arbitrary nondecreasing out_off
+arrays (empty blocks, gaps, a nonzero first offset), a directory shorter
+than the stream, and the directory's own entries against numpy."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from utlpfor_amd.device import Context
+    return Context(0)
+
+
+def _streams(seed, nstreams, top, is64=False):
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0, np.log(top + 1), nstreams))).astype(np.int64)
+    flags = rng.choice([0, 1, 2, 4, 3], lens.size).astype(np.uint8)
+    parts = []
+    for L in lens:
+        if rng.random() < 0.5:
+            parts.append(rng.integers(0, 1 << int(rng.integers(0, 33)), int(L), dtype=np.uint64))
+        else:
+            parts.append((np.cumsum(rng.integers(-1024, 1025, int(L))) + 7000).astype(np.int64).astype(np.uint64))
+    v = np.concatenate(parts)
+    return (v if is64 else v.astype(np.uint32)), lens, flags
+
+
+def _get(ctx, lib, s, pos, d=None, ndir=None):
+    """Raw C-ABI calls: with the directory d (ndir entries) or without."""
+    from utlpfor_amd.device import DeviceArray, check
+    dp = DeviceArray.from_numpy(pos, ctx)
+    out = DeviceArray(max(pos.size, 1), np.uint64 if s.is64 else np.uint32, ctx)
+    st = DeviceArray(max(pos.size, 1), np.int32, ctx)
+    if d is None:
+        check(lib.utl_get_batch(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
+                                None, dp.ptr, pos.size, out.ptr, st.ptr), "get")
+    else:
+        check(lib.utl_get_batch_dir(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
+                                    d.ptr, ndir, dp.ptr, pos.size, out.ptr, st.ptr), "get_dir")
+    ctx.sync()
+    r = out.to_numpy()[:pos.size], st.to_numpy()[:pos.size]
+    for a in (dp, out, st):
+        a.free()
+    return r
+
+
+def _dir(ctx, lib, ooff_dev, nblocks, ndir):
+    from utlpfor_amd.device import DeviceArray, check
+    d = DeviceArray(ndir, np.uint64, ctx)
+    check(lib.utl_pos_dir_build(ctx.handle, ooff_dev.ptr, nblocks, d.ptr, ndir), "pos_dir")
+    ctx.sync()
+    return d
+
+
+def _want_dir(ooff, ndir):
+    c = np.arange(ndir, dtype=np.uint64) * 128
+    return np.maximum(np.searchsorted(ooff[:-1], c, side="right") - 1, 0).astype(np.uint64)
+
+
+@pytest.mark.parametrize("is64", [False, True])
+def test_dir_get_equals_search_and_values(ctx, is64):
+    from utlpfor_amd import _lib, batch
+    lib = _lib.load()
+    v, lens, flags = _streams(31 + is64, 3000, 5000, is64)
+    say(f"dir get: {lens.size} streams, {v.size} values, u64={is64}")
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    ooff = s.out_off.to_numpy()
+    total = int(ooff[-1])
+    assert total == v.size
+    ndir = (total + 127) // 128
+    d = s.ensure_pos_dir()
+    assert d.n == ndir and np.array_equal(d.to_numpy(), _want_dir(ooff, ndir))
+    rng = np.random.default_rng(5)
+    edges = np.concatenate([ooff[:-1], ooff[1:] - 1, [0, total - 1, -1, -7, total, total + 1, total + 300]])
+    pos = np.concatenate([rng.integers(0, total, 200000), edges.astype(np.int64)]).astype(np.int64)
+    g_dir, s_dir = _get(ctx, lib, s, pos, d, ndir)
+    g_srch, s_srch = _get(ctx, lib, s, pos)
+    assert np.array_equal(s_dir, s_srch) and np.array_equal(g_dir, g_srch)
+    ok = (pos >= 0) & (pos < total)
+    # statuses: OK inside the stream (the u64 two-block FOR quirk aside),
+    # PositionOutOfRange outside
+    assert np.all(s_dir[~ok] == 2)
+    good = ok & (s_dir == 0)
+    assert good.sum() >= 0.5 * ok.sum()
+    assert np.array_equal(g_dir[good], v[pos[good]])
+    # the Python wrapper takes the directory by default
+    vals, gst = batch.get_batch(ctx, s, batch.DeviceArray.from_numpy(pos, ctx))
+    ctx.sync()
+    assert np.array_equal(vals.to_numpy(), g_dir) and np.array_equal(gst.to_numpy(), s_dir)
+    s.free()
+
+
+def test_dir_over_arbitrary_out_off(ctx):
+    """out_off arrays a caller may pass: empty blocks (repeated entries), gaps
+    after blocks, a nonzero first entry, long runs of tiny blocks; and
+    directories shorter and longer than the stream. Block contents are the
+    real stream's, so each search's block gives a value or a status, and
+    the two searches must agree on every position."""
+    from utlpfor_amd import _lib, batch
+    from utlpfor_amd.device import DeviceArray
+    lib = _lib.load()
+    v, lens, flags = _streams(41, 400, 3000)
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    counts = np.diff(s.out_off.to_numpy()).astype(np.int64)
+    rng = np.random.default_rng(9)
+    for trial in range(4):
+        c = counts.copy()
+        c[rng.random(c.size) < 0.15] = 0                       # empty blocks
+        c += np.where(rng.random(c.size) < 0.1, rng.integers(1, 200, c.size), 0)  # gaps
+        if trial == 3:
+            c[: c.size // 2] = np.minimum(c[: c.size // 2], 2)  # runs of tiny blocks
+        first = int(rng.integers(0, 300)) if trial >= 1 else 0
+        ooff = np.concatenate([[first], first + np.cumsum(c)]).astype(np.uint64)
+        total = int(ooff[-1])
+        s2 = batch.DeviceStream(ctx, s.data, s.meta, s.block_off, DeviceArray.from_numpy(ooff, ctx), s.nblocks,
+                                total, s.nbytes, False)
+        pos = np.concatenate([rng.integers(-5, total + 5, 60000), ooff.astype(np.int64),
+                              ooff.astype(np.int64) - 1]).astype(np.int64)
+        g_srch, s_srch = _get(ctx, lib, s2, pos)
+        for ndir in ((total + 127) // 128, 1, max((total + 127) // 128 // 3, 1), (total + 127) // 128 + 50):
+            say(f"trial {trial}: first {first}, total {total}, ndir {ndir}")
+            d = _dir(ctx, lib, s2.out_off, s2.nblocks, ndir)
+            assert np.array_equal(d.to_numpy(), _want_dir(ooff, ndir))
+            g_dir, s_dir = _get(ctx, lib, s2, pos, d, ndir)
+            assert np.array_equal(s_dir, s_srch) and np.array_equal(g_dir, g_srch)
+            d.free()
+        s2.out_off.free()
+    s.free()

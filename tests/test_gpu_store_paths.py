@@ -147,15 +147,18 @@ def test_slots_wider_than_blocks(ctx):
         assert np.all(got[a + c:int(wide[k + 1])] == POISON), k
     # Get by global position over the same index: a position in a block's
     # slot reads its value; one in the gap after it is past that block's
-    # count (PositionOutOfRange). The block search starts at pos / 128,
-    # which overshoots here and must walk back.
+    # count (PositionOutOfRange). The directory-less search starts at
+    # pos / 128, which overshoots here and must walk back.
     pos = rng.integers(0, int(wide[-1]), 20000).astype(np.int64)
     blk = np.searchsorted(wide, pos, side="right") - 1
     inb = pos - wide[blk].astype(np.int64)
     valid = inb < counts[blk].astype(np.int64)
-    vals, gst = batch.get_batch(ctx, s2, DeviceArray.from_numpy(pos, ctx))
-    ctx.sync()
-    gv, gs = vals.to_numpy(), gst.to_numpy()
-    assert np.all(gs[valid] == 0) and np.all(gs[~valid] == 2)  # ErrPositionOutOfRange
-    assert np.array_equal(gv[valid], v[(ooff[blk] + inb)[valid].astype(np.int64)])
+    for use_dir in (True, False):
+        vals, gst = batch.get_batch(ctx, s2, DeviceArray.from_numpy(pos, ctx), use_dir=use_dir)
+        ctx.sync()
+        gv, gs = vals.to_numpy(), gst.to_numpy()
+        assert np.all(gs[valid] == 0) and np.all(gs[~valid] == 2)  # ErrPositionOutOfRange
+        assert np.array_equal(gv[valid], v[(ooff[blk] + inb)[valid].astype(np.int64)])
+    s2.pos_dir.free()
+    s2.out_off.free()
     s.free()

@@ -32,13 +32,28 @@ class DeviceStream:
     nvalues: int
     nbytes: int
     is64: bool
+    pos_dir: DeviceArray | None = None  # position directory, built by the first Get by stream position
 
     def bytes_host(self) -> np.ndarray:
         return self.data.to_numpy(self.nbytes)
 
+    def ensure_pos_dir(self) -> DeviceArray:
+        """The position directory over out_off (utl_pos_dir_build): entry c is
+        the block holding value position 128c."""
+        if self.pos_dir is None:
+            lib = _lib.load()
+            ndir = max((self.nvalues + 127) // 128, 1)
+            d = DeviceArray(ndir, np.uint64, self.ctx)
+            check(lib.utl_pos_dir_build(self.ctx.handle, self.out_off.ptr, self.nblocks, d.ptr, ndir),
+                  "utl_pos_dir_build")
+            self.pos_dir = d
+        return self.pos_dir
+
     def free(self) -> None:
-        for a in (self.data, self.meta, self.block_off, self.out_off):
-            a.free()
+        for a in (self.data, self.meta, self.block_off, self.out_off, self.pos_dir):
+            if a is not None:
+                a.free()
+        self.pos_dir = None
 
 
 def plan_blocks(stream_lengths: np.ndarray) -> np.ndarray:
@@ -227,14 +242,21 @@ def decode_meta(ctx: Context, s: DeviceStream, wave: DeviceArray, out: DeviceArr
 
 
 def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: DeviceArray | None = None,
-              status: DeviceArray | None = None, blocks: DeviceArray | None = None):
-    """Batched Get of stream positions (or of (block, in-block pos) pairs)."""
+              status: DeviceArray | None = None, blocks: DeviceArray | None = None, use_dir: bool = True):
+    """Batched Get of stream positions (or of (block, in-block pos) pairs).
+    Stream positions go through the stream's position directory
+    (utl_get_batch_dir) unless use_dir is False (utl_get_batch's search)."""
     lib = _lib.load()
     n = positions.n
     if out is None:
         out = DeviceArray(max(n, 1), np.uint64 if s.is64 else np.uint32, ctx)
     if status is None:
         status = DeviceArray(max(n, 1), np.int32, ctx)
+    if blocks is None and use_dir and s.nblocks > 0:
+        d = s.ensure_pos_dir()
+        check(lib.utl_get_batch_dir(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
+                                    d.ptr, d.n, positions.ptr, n, out.ptr, status.ptr), "utl_get_batch_dir")
+        return out, status
     check(lib.utl_get_batch(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
                             blocks.ptr if blocks is not None else None, positions.ptr, n, out.ptr, status.ptr),
           "utl_get_batch")

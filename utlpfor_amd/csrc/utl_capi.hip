@@ -299,8 +299,8 @@ static int get_call(utl_ctx* ctx, bool u64, int64_t pos, const uint8_t* src, siz
   uint8_t* d = c->d_stage;
   int32_t* d_status = (int32_t*)(d + kResultOff);
   void* d_val = d + kResultOff + 64;
-  HIP_TRY(utl_launch_get(u64, d + 64, (const uint64_t*)d, (const uint64_t*)(d + 16), 1, (const uint64_t*)(d + 32),
-                         (const int64_t*)(d + 40), 1, d_val, d_status, c->stream));
+  HIP_TRY(utl_launch_get(u64, d + 64, (const uint64_t*)d, (const uint64_t*)(d + 16), 1, nullptr, 0,
+                         (const uint64_t*)(d + 32), (const int64_t*)(d + 40), 1, d_val, d_status, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int32_t st = *(volatile int32_t*)(c->h_stage + kResultOff);
   if (u64) *(uint64_t*)out = st == UTL_OK ? *(uint64_t*)(c->h_stage + kResultOff + 64) : 0;
@@ -621,8 +621,30 @@ int utl_get_batch(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const uint64_t
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_get(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_req_block, d_req_pos, nreq, d_out,
-                         d_status, c->stream));
+  HIP_TRY(utl_launch_get(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, nullptr, 0, d_req_block, d_req_pos,
+                         nreq, d_out, d_status, c->stream));
+  return UTL_OK;
+}
+
+int utl_pos_dir_build(utl_ctx* ctx, const uint64_t* d_out_off, size_t nblocks, uint64_t* d_dir, size_t ndir) {
+  if (ndir && (!d_out_off || !d_dir || nblocks == 0)) return UTL_ERR_ARGUMENT;
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_pos_dir(d_out_off, nblocks, d_dir, ndir, c->stream));
+  return UTL_OK;
+}
+
+int utl_get_batch_dir(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const uint64_t* d_block_off,
+                      const uint64_t* d_out_off, size_t nblocks, const uint64_t* d_dir, size_t ndir,
+                      const int64_t* d_req_pos, size_t nreq, void* d_out, int32_t* d_status) {
+  if (nreq && (!d_src || !d_block_off || !d_out_off || !d_dir || !ndir || !d_req_pos || !d_out || !d_status))
+    return UTL_ERR_ARGUMENT;
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_get(is_u64 != 0, d_src, d_block_off, d_out_off, nblocks, d_dir, ndir, nullptr, d_req_pos, nreq,
+                         d_out, d_status, c->stream));
   return UTL_OK;
 }
 

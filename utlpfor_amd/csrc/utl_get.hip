@@ -337,12 +337,15 @@ __device__ int get_u64(int64_t pos, const uint8_t* src, int64_t len, uint64_t* o
   return UTL_OK;
 }
 
-// Requests: either (req_block[i], req_pos[i]) pairs, or global value
-// positions req_pos[i] that are mapped to their block by binary search over
-// out_off[0..nblocks].
+// Requests: either (req_block[i], req_pos[i]) pairs, or stream value
+// positions req_pos[i], each mapped to the last block b with ooff[b] <= pos.
+// With a position directory (dir != nullptr, see k_pos_dir) the search
+// starts at dir[pos / 128], the block holding position pos & ~127, so it
+// usually reads one out_off line; without one it gallops from pos / 128.
 template <bool U64>
 __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                              const uint64_t* __restrict__ ooff, uint64_t nblocks,
+                                             const uint64_t* __restrict__ dir, uint64_t ndir,
                                              const uint64_t* __restrict__ req_block, const int64_t* __restrict__ req_pos,
                                              uint64_t nreq, void* __restrict__ out, int32_t* __restrict__ status) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -355,16 +358,16 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
     st = b < nblocks ? UTL_OK : UTL_ERR_ARGUMENT;
   } else {
     st = (pos >= 0 && (uint64_t)pos < ooff[nblocks]) ? UTL_OK : UTL_ERR_POSITION_OUT_OF_RANGE;
-    // Largest b with ooff[b] <= pos. Blocks of an index built from meta
-    // words hold at most 128 values, so b >= pos / 128, and b == pos / 128
-    // when the blocks before are full: gallop from there to a bracket
-    // [lo, hi) with ooff[lo] <= pos < ooff[hi] (hi == nblocks: past the
-    // end), then bisect. Any nondecreasing out_off gives the same b as a
-    // bisection of the whole array.
+    // Largest b with ooff[b] <= pos: a bracket [lo, hi) with ooff[lo] <=
+    // pos < ooff[hi] (hi == nblocks: past the end; lo == 0 also when no
+    // entry is <= pos), then bisection. Any nondecreasing out_off gives the
+    // same b as a bisection of the whole array.
     uint64_t lo = 0, hi = nblocks;
-    const uint64_t g = st == UTL_OK ? min((uint64_t)pos >> 7, nblocks - 1) : 0;
-    if (ooff[g] <= (uint64_t)pos) {
-      lo = g;
+    if (dir) {
+      // dir[c] is the largest b with ooff[b] <= 128c (or 0), and 128c <= pos:
+      // gallop up from it
+      const uint64_t c = st == UTL_OK ? (uint64_t)pos >> 7 : 0;
+      lo = c < ndir ? min(dir[c], nblocks - 1) : 0;
       for (uint64_t step = 1; lo + step < nblocks; step <<= 1) {
         if (ooff[lo + step] > (uint64_t)pos) {
           hi = lo + step;
@@ -373,13 +376,28 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
         lo += step;
       }
     } else {
-      hi = g;
-      for (uint64_t step = 1; step <= hi; step <<= 1) {
-        if (ooff[hi - step] <= (uint64_t)pos) {
-          lo = hi - step;
-          break;
+      // Blocks of an index built from meta words hold at most 128 values,
+      // so b >= pos / 128, and b == pos / 128 when the blocks before are
+      // full: gallop from there.
+      const uint64_t g = st == UTL_OK ? min((uint64_t)pos >> 7, nblocks - 1) : 0;
+      if (ooff[g] <= (uint64_t)pos) {
+        lo = g;
+        for (uint64_t step = 1; lo + step < nblocks; step <<= 1) {
+          if (ooff[lo + step] > (uint64_t)pos) {
+            hi = lo + step;
+            break;
+          }
+          lo += step;
         }
-        hi -= step;
+      } else {
+        hi = g;
+        for (uint64_t step = 1; step <= hi; step <<= 1) {
+          if (ooff[hi - step] <= (uint64_t)pos) {
+            lo = hi - step;
+            break;
+          }
+          hi -= step;
+        }
       }
     }
     while (hi - lo > 1) {
@@ -409,16 +427,40 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
   status[i] = st;
 }
 
+// Position directory: dir[c] = the largest block b with ooff[b] <= 128c
+// (0 when there is none), for c < ndir. Thread b writes the entries c with
+// 128c in [ooff[b], ooff[b + 1]), widened to [0, ooff[1]) for block 0 and
+// to [ooff[b], 128 ndir) for the last block; with a nondecreasing ooff these
+// ranges tile [0, 128 ndir), so every entry is written exactly once, and
+// the block of each range is the largest whose offset is <= any position in
+// it (block 0 also where no offset is).
+__global__ __launch_bounds__(256) void k_pos_dir(const uint64_t* __restrict__ ooff, uint64_t nblocks,
+                                                 uint64_t* __restrict__ dir, uint64_t ndir) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const uint64_t lo = b == 0 ? 0 : ooff[b], hi = b + 1 == nblocks ? (ndir << 7) : ooff[b + 1];
+  for (uint64_t c = (lo + 127) >> 7; c < ndir && (c << 7) < hi; ++c) dir[c] = b;
+}
+
 }  // namespace utl
 
 using namespace utl;
 
 hipError_t utl_launch_get(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
-                          const uint64_t* req_block, const int64_t* req_pos, uint64_t nreq, void* out, int32_t* status,
-                          hipStream_t stream) {
+                          const uint64_t* dir, uint64_t ndir, const uint64_t* req_block, const int64_t* req_pos,
+                          uint64_t nreq, void* out, int32_t* status, hipStream_t stream) {
   if (nreq == 0) return hipSuccess;
   const dim3 grid((unsigned)((nreq + 255) / 256)), block(256);
-  if (u64) k_get<true><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, req_block, req_pos, nreq, out, status);
-  else k_get<false><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, req_block, req_pos, nreq, out, status);
+  if (u64)
+    k_get<true><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, dir, ndir, req_block, req_pos, nreq, out, status);
+  else
+    k_get<false><<<grid, block, 0, stream>>>(src, boff, ooff, nblocks, dir, ndir, req_block, req_pos, nreq, out, status);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_pos_dir(const uint64_t* ooff, uint64_t nblocks, uint64_t* dir, uint64_t ndir,
+                              hipStream_t stream) {
+  if (nblocks == 0 || ndir == 0) return hipSuccess;
+  k_pos_dir<<<dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, stream>>>(ooff, nblocks, dir, ndir);
   return hipGetLastError();
 }
