@@ -139,3 +139,87 @@ def test_dir_over_arbitrary_out_off(ctx):
             d.free()
         s2.out_off.free()
     s.free()
+
+
+def _fallback_blocks(orc, rng, is64, nblocks=500):
+    """Delta blocks whose Gets past row 4 take the full-unpack fallback
+    (widths >= 20): zigzag deltas of a wide random walk, increasing values
+    with wide steps (plain delta), and both with outlier steps (exception
+    lists up to 16, bitmaps above), steps near 2^31 (32-bit values that wrap,
+    so zigzag), full and partial; then damaged copies: the zigzag flag
+    cleared (per-lane sums of zigzag words, which wrap for the near-2^31
+    steps: ErrOverflow) or a byte changed."""
+    blobs = []
+    for k in range(nblocks):
+        n = 128 if rng.random() < 0.7 else int(rng.integers(1, 129))
+        kind = k % 5
+        if kind == 0:
+            v = np.cumsum(rng.integers(-(1 << 20), 1 << 20, n)) + (1 << 40)
+        elif kind == 4:
+            v = np.cumsum(rng.integers(1 << 29, 1 << 31, n))
+        else:
+            steps = rng.integers(1 << 19, 1 << 21, n)
+            if kind == 2:
+                idx = rng.choice(n, min(n, int(rng.integers(1, 16))), replace=False)
+                steps[idx] = rng.integers(1 << 26, 1 << 28, idx.size)
+            elif kind == 3:
+                m = rng.random(n) < 0.3
+                steps[m] = rng.integers(1 << 25, 1 << 27, int(m.sum()))
+            v = np.cumsum(steps)
+        v = v.astype(np.int64).astype(np.uint64)
+        flag = 1 if k % 2 == 0 else 3  # Delta, Delta|NoFOR
+        b = orc.PackUint64(flag, v)[0] if is64 else orc.PackUint32(flag, v.astype(np.uint32))[0]
+        blobs.append(np.frombuffer(bytes(b), np.uint8).copy())
+    out = list(blobs)
+    for b in blobs:
+        h = int.from_bytes(bytes(b[:4]), "little")
+        if h & (1 << 23):  # zigzag -> plain delta over the same words
+            d = b.copy()
+            d[2] &= 0x7F
+            out.append(d)
+        d = b.copy()
+        d[int(rng.integers(4, d.size))] ^= int(rng.integers(1, 256))
+        out.append(d)
+    return out
+
+
+@pytest.mark.parametrize("is64", [False, True])
+def test_full_unpack_fallback(ctx, orc, is64):
+    """Every position (and three out of range) of blocks whose Gets take the
+    full-unpack fallback, which runs on 16-lane groups: values and statuses
+    against the oracle's GetUint32 / GetUint64, and ErrOverflow's position
+    against the oracle's UnpackUint32 of the block."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(88 + is64)
+    blobs = _fallback_blocks(orc, rng, is64)
+    data = np.concatenate(blobs)
+    lens = np.array([b.size for b in blobs], np.int64)
+    counts = np.array([b[0] for b in blobs], np.int64)
+    cs = batch.from_host_blocks(ctx, data, lens, counts, is64=is64)
+    blk, pos = [], []
+    for k, b in enumerate(blobs):
+        c = int(b[0])
+        p = list(range(c)) + [-1, c, 200]
+        blk += [k] * len(p)
+        pos += p
+    blk, pos = np.array(blk, np.uint64), np.array(pos, np.int64)
+    say(f"fallback gets: {len(blobs)} blocks, {pos.size} Gets, u64={is64}")
+    vals, gst = batch.get_batch(ctx, cs, DeviceArray.from_numpy(pos, ctx), blocks=DeviceArray.from_numpy(blk, ctx))
+    ctx.sync()
+    gv, gs = vals.to_numpy(), gst.to_numpy()
+    get = orc.GetUint64 if is64 else orc.GetUint32
+    seen = {0: 0, 6: 0}
+    for i in range(pos.size):
+        b = bytes(blobs[int(blk[i])])
+        wv, we = get(int(pos[i]), b)
+        assert (gs[i] & 0xFF) == (1 if we == 99 else we), (int(blk[i]), int(pos[i]), we, int(gs[i]))
+        if we == 0:
+            assert int(gv[i]) == wv, (int(blk[i]), int(pos[i]))
+        elif we == 6 and not is64:
+            assert (gs[i] >> 8) == orc.UnpackUint32(b)[3], (int(blk[i]), int(pos[i]))
+        if we in seen and pos[i] >= 80:
+            seen[we] += 1
+    # the fallback ran, on valid blocks and on overflowing ones
+    assert seen[0] > 1000 and seen[6] > (100 if not is64 else 20), seen
+    cs.free()
