@@ -316,6 +316,21 @@ int utl_encode_blocks(utl_ctx *ctx, int is_u64, const void *d_values, const uint
 int utl_encode_u32_direct(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *d_in_off,
                           const uint32_t *d_in_meta, size_t nblocks, uint8_t *d_dst, size_t dst_cap,
                           uint32_t *d_out_meta, uint64_t *d_block_off, size_t *out_nbytes);
+/* Batched PackUint32 in two passes with the index scan between them (no
+ * slots, no compaction, no waits between workgroups). Same job table as
+ * utl_encode_blocks. utl_encode_u32_size makes every block's decisions
+ * (FOR, zigzag, bit width, exception count: packBlockScalar, scalar.go:126-235)
+ * and writes d_out_meta[b] (length | count << 24, or 0xFFFFFFFF for a count
+ * outside 1..128) and d_plan[b] (8 bytes: the decisions). The caller scans
+ * d_out_meta (utl_index_build); utl_encode_u32_write then packs block b at
+ * d_block_off[b] of d_dst. d_dst, d_out_meta and d_block_off receive what
+ * utl_encode_blocks + utl_index_build + utl_compact give, byte for byte. A
+ * block that would end past dst_cap is not written. */
+int utl_encode_u32_size(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *d_in_off,
+                        const uint32_t *d_in_meta, size_t nblocks, uint32_t *d_out_meta, uint64_t *d_plan);
+int utl_encode_u32_write(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *d_in_off,
+                         const uint32_t *d_in_meta, size_t nblocks, const uint64_t *d_plan,
+                         const uint64_t *d_block_off, uint8_t *d_dst, size_t dst_cap);
 /* Concatenates encoded slots into d_dst at d_block_off (from utl_index_build
  * over d_out_meta): byte-identical to appending the blocks with Append.
  * Writes exactly the stream's d_block_off[nblocks] bytes. */

@@ -35,7 +35,7 @@ def _meta_of(bl, bc):
     return (bl.astype(np.uint32) & 0xFFFFFF) | (bc.astype(np.uint32) << 24)
 
 
-@pytest.mark.parametrize("method", ["slot", "direct"])
+@pytest.mark.parametrize("method", ["slot", "direct", "sized"])
 def test_delta64m_encode_bytes_and_decode(ctx, orc, method):
     from utlpfor_amd import batch, workloads
     v = workloads.delta_postings(64 << 20)
@@ -54,7 +54,7 @@ def test_delta64m_encode_bytes_and_decode(ctx, orc, method):
     s.free()
 
 
-@pytest.mark.parametrize("method", ["slot", "direct"])
+@pytest.mark.parametrize("method", ["slot", "direct", "sized"])
 def test_uneven_20000_streams_encode_bytes(ctx, orc, method):
     from utlpfor_amd import batch, workloads
     v, lens, flags = workloads.uneven_streams(20_000, seed=3)

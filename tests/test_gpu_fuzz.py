@@ -265,7 +265,7 @@ def _edge_values(rng, n, is64):
 
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("is64", [False, True])
-@pytest.mark.parametrize("method", ["slot", "direct"])
+@pytest.mark.parametrize("method", ["slot", "direct", "sized"])
 def test_fuzz_encode_bytes(ctx, orc, seed, is64, method):
     """GPU encode of many short streams with every flag combination, in both
     encode forms (uint64 has the slot form only): the packed bytes equal the
@@ -289,3 +289,64 @@ def test_fuzz_encode_bytes(ctx, orc, seed, is64, method):
         b += bl.size
     assert off == s.nbytes and b == s.nblocks
     s.free()
+
+
+def test_encode_forms_with_invalid_jobs(ctx, orc):
+    """A job table with counts outside 1..128 (ErrInvalidBuffer: meta word
+    0xFFFFFFFF, nothing written) between valid jobs: the slot and sized
+    uint32 forms give the same meta words, and every valid block's bytes at
+    its scanned offset in the sized form's stream equal the oracle's
+    PackUint32 of its values (the slot form's compaction copies whole tiles,
+    so the blocks after a rejected one in its tile are not placed)."""
+    from utlpfor_amd import _lib, batch
+    from utlpfor_amd.device import DeviceArray, check
+    lib = _lib.load()
+    rng = np.random.default_rng(4242)
+    counts = rng.integers(1, 129, 300)
+    counts[[3, 77, 150, 299]] = [0, 129, 200, 255]
+    flags = rng.choice(8, counts.size).astype(np.uint32)
+    vals = _edge_values(rng, int(counts.sum()) + 300, False)
+    in_off = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint64)
+    in_meta = (counts.astype(np.uint32) & 0xFF) | (flags << 8)
+    nb = counts.size
+    dv, doff, dmeta = (DeviceArray.from_numpy(a, ctx) for a in (vals, in_off, in_meta))
+    metas, streams = {}, {}
+    for form in ("slot", "sized"):
+        meta = DeviceArray(nb, np.uint32, ctx)
+        if form == "slot":
+            slots = DeviceArray(nb * batch.SLOT32, np.uint8, ctx)
+            check(lib.utl_encode_blocks(ctx.handle, 0, dv.ptr, doff.ptr, dmeta.ptr, nb, slots.ptr, meta.ptr), "enc")
+        else:
+            plan = DeviceArray(nb, np.uint64, ctx)
+            check(lib.utl_encode_u32_size(ctx.handle, dv.ptr, doff.ptr, dmeta.ptr, nb, meta.ptr, plan.ptr), "size")
+        boff, ooff = batch.build_index(ctx, meta, nb)
+        total = int(boff.read(nb, 1)[0])
+        dst = DeviceArray(total + 64, np.uint8, ctx)
+        if form == "slot":
+            check(lib.utl_compact(ctx.handle, 0, slots.ptr, boff.ptr, nb, dst.ptr), "compact")
+            slots.free()
+        else:
+            check(lib.utl_encode_u32_write(ctx.handle, dv.ptr, doff.ptr, dmeta.ptr, nb, plan.ptr, boff.ptr, dst.ptr,
+                                           dst.nbytes), "write")
+            plan.free()
+        ctx.sync()
+        metas[form], bo, data = meta.to_numpy(), boff.to_numpy(), dst.to_numpy()
+        streams[form] = []
+        for k in range(nb):
+            if 1 <= counts[k] <= 128:
+                streams[form].append(data[int(bo[k]):int(bo[k + 1])].copy())
+        for a in (meta, boff, ooff, dst):
+            a.free()
+    assert np.array_equal(metas["slot"], metas["sized"])
+    bad = (counts < 1) | (counts > 128)
+    assert np.all(metas["sized"][bad] == 0xFFFFFFFF)
+    i = 0
+    for k in range(nb):
+        if bad[k]:
+            continue
+        v = vals[int(in_off[k]):int(in_off[k]) + int(counts[k])]
+        want = np.frombuffer(bytes(orc.PackUint32(int(flags[k]) | orc.NoInPlace, v.copy())[0]), np.uint8)
+        assert np.array_equal(streams["sized"][i], want), k
+        i += 1
+    for a in (dv, doff, dmeta):
+        a.free()

@@ -62,6 +62,8 @@ SIGNATURES = {
     "utl_index_walk": (C.c_int, [vp, vp, vp, sz, vp, vp, vp, vp, vp]),
     "utl_encode_blocks": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp]),
     "utl_compact": (C.c_int, [vp, C.c_int, vp, vp, sz, vp]),
+    "utl_encode_u32_size": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),
+    "utl_encode_u32_write": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, vp, sz]),
     "utl_encode_u32_direct": (C.c_int, [vp, vp, vp, vp, sz, vp, sz, vp, vp, C.POINTER(sz)]),
     "utl_make_jobs": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),
     "utl_graph_decode_create": (vp, [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, vp]),

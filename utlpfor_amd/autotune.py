@@ -160,7 +160,7 @@ def tune_decode(ctx: Context, s: batch.DeviceStream, out: DeviceArray | None = N
     return dict(rec, candidates=results, cached=False)
 
 
-# ---- encode form: slots + compaction, or direct writes ----
+# ---- encode form: slots + compaction, direct writes, or size / scan / write ----
 
 def _encode_key(ctx: Context, nblocks: int, nvalues: int) -> str:
     name, cus = ctx.device_info()
@@ -175,11 +175,10 @@ def cached_encode_method(ctx: Context, nblocks: int, nvalues: int):
 
 def tune_encode(ctx: Context, values: DeviceArray, stream_lengths=None, stream_flags=None, flag: int = 0,
                 reps: int = 3, use_cache: bool = True) -> dict:
-    """Times both uint32 encode forms on the caller's resident values (the
-    whole batch.encode: job table, encode, index, compaction for the slot
-    form), in turn, keeps the faster (median of reps) and caches the choice
-    for batch.encode(method="auto"). Returns {"method", "slot_ms",
-    "direct_ms"}."""
+    """Times every uint32 encode form on the caller's resident values (the
+    whole batch.encode: job table, kernels, index scan, allocations), in
+    turn, keeps the fastest (median of reps) and caches the choice for
+    batch.encode(method="auto"). Returns {"method", "<form>_ms" for each}."""
     import time
     times = {m: [] for m in batch.ENCODE_METHODS}
     nblocks = 0
@@ -193,7 +192,8 @@ def tune_encode(ctx: Context, values: DeviceArray, stream_lengths=None, stream_f
             nblocks = s.nblocks
             s.free()
     med = {m: float(np.median(t[1:])) for m, t in times.items()}  # the first round warms up
-    rec = {"method": min(med, key=med.get), "slot_ms": med["slot"], "direct_ms": med["direct"]}
+    rec = {"method": min(med, key=med.get)}
+    rec.update({f"{m}_ms": t for m, t in med.items()})
     if use_cache:
         cache = _load()
         cache[_encode_key(ctx, nblocks, values.n)] = rec

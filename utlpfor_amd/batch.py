@@ -70,22 +70,22 @@ def build_index(ctx: Context, meta: DeviceArray, nblocks: int):
     return boff, ooff
 
 
-ENCODE_METHODS = ("slot", "direct")
+ENCODE_METHODS = ("slot", "direct", "sized")
 
 
 def encode_method(ctx: Context, is64: bool, nblocks: int, nvalues: int, method: str = "auto") -> str:
     """The encode form to use: "slot" (each block into a fixed slot, then the
-    index scan and a compaction pass) or "direct" (uint32 only: blocks
-    written straight to their place, offsets by look-back). "auto" takes the
-    autotuner's cached choice for this shape of work, else the slot form
-    (the faster on MI355X for the 64 Mi delta postings and for 200,000
-    uneven streams: 0.21 vs 0.40 ms and 7.9 vs 17.0 ms of GPU time)."""
+    index scan and a compaction pass), "direct" (uint32 only: blocks
+    written straight to their place, offsets by look-back) or "sized"
+    (uint32 only: a size pass, the index scan, a write pass at the
+    offsets). "auto" takes the autotuner's cached choice for this shape of
+    work, else the sized form for uint32 and the slot form for uint64."""
     if is64:
         return "slot"
     if method in ENCODE_METHODS:
         return method
     from . import autotune
-    return autotune.cached_encode_method(ctx, nblocks, nvalues) or "slot"
+    return autotune.cached_encode_method(ctx, nblocks, nvalues) or "sized"
 
 
 def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flags=None,
@@ -114,7 +114,19 @@ def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flag
     check(lib.utl_make_jobs(ctx.handle, d_val_off.ptr, d_blk_off.ptr, d_flags.ptr, stream_lengths.size, in_off.ptr,
                             in_meta.ptr), "utl_make_jobs")
     meta = DeviceArray(nblocks, np.uint32, ctx)
-    if encode_method(ctx, bool(is64), nblocks, n, method) == "direct":
+    form = encode_method(ctx, bool(is64), nblocks, n, method)
+    if form == "sized":
+        # decisions and lengths, the index scan, then every block at its place
+        plan = DeviceArray(max(nblocks, 1), np.uint64, ctx)
+        check(lib.utl_encode_u32_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr, plan.ptr),
+              "utl_encode_u32_size")
+        boff, ooff = build_index(ctx, meta, nblocks)
+        total = int(boff.read(nblocks, 1)[0]) if nblocks else 0
+        data = DeviceArray(total + 64, np.uint8, ctx)
+        check(lib.utl_encode_u32_write(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, plan.ptr, boff.ptr,
+                                       data.ptr, data.nbytes), "utl_encode_u32_write")
+        scratch = (plan,)
+    elif form == "direct":
         # the stream written in place; the index scan then gives both offsets
         data = DeviceArray(nblocks * MAX_BLOCK32 + 64, np.uint8, ctx)
         dboff = DeviceArray(nblocks + 1, np.uint64, ctx)

@@ -21,6 +21,15 @@ size_t utl_encode_direct_tmp_bytes(uint64_t nblocks);
 hipError_t utl_launch_encode_u32_direct(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                         uint64_t nblocks, uint8_t* dst, uint64_t dst_cap, uint32_t* out_meta,
                                         uint64_t* block_off, uint32_t* work_out, uint64_t* lb, hipStream_t stream);
+// Sized form of the u32 encode: the size pass writes out_meta and plan (8
+// bytes a block); after the caller's index scan of out_meta, the write pass
+// packs block b at block_off[b] of dst (blocks ending past dst_cap are not
+// written). Byte-identical to the slot form + compaction.
+hipError_t utl_launch_encode_u32_size(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                      uint64_t nblocks, uint32_t* out_meta, uint64_t* plan, hipStream_t stream);
+hipError_t utl_launch_encode_u32_write(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                       uint64_t nblocks, const uint64_t* plan, const uint64_t* block_off, uint8_t* dst,
+                                       uint64_t dst_cap, hipStream_t stream);
 hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  hipStream_t stream);

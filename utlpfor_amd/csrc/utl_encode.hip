@@ -668,6 +668,189 @@ __global__ __launch_bounds__(64 * kEncGrpWaves, 4) void k_encode_u32(const uint3
                                    slot_bytes, out_meta, work_out, lb, block_off, dst_cap);
 }
 
+// ---- the sized form of the uint32 encode: a size pass, the index scan, a write pass ----
+//
+// The size pass makes every decision of packBlockScalar for a block (FOR,
+// zigzag, bit width, exception count) and its length, from which the
+// StreamVByte data bytes follow (a byte per exception plus its code). It
+// writes the meta word and an 8-byte plan: bw | FOR width code << 8 |
+// zigzag << 10 | exception count << 16, then the FOR base. The caller scans
+// the meta words (the block offsets); the write pass reloads the values,
+// applies the planned FOR and delta, and packs the block at its offset.
+// Nothing is staged or copied, and neither pass waits on another workgroup.
+
+__device__ __forceinline__ void enc_load(const uint32_t* __restrict__ v, int n, int j, bool full, uint32_t* x0,
+                                         uint32_t* x1, bool* ok0, bool* ok1) {
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int p = 16 * r + 2 * j;
+    ok0[r] = full || p < n;
+    ok1[r] = full || p + 1 < n;
+    x0[r] = ok0[r] ? v[p] : 0u;
+    x1[r] = ok1[r] ? v[p + 1] : 0u;
+  }
+}
+
+template <bool FULL>
+__device__ __forceinline__ void size_u32_wave(uint64_t b, bool mine, bool ok, int n, uint32_t flag, uint64_t voff,
+                                              const uint32_t* __restrict__ values, uint32_t* __restrict__ out_meta,
+                                              uint2* __restrict__ plan) {
+  const int j = lane_id() & 7;
+  uint32_t x0[kRows], x1[kRows];
+  bool ok0[kRows], ok1[kRows];
+  enc_load(values + voff, n, j, FULL, x0, x1, ok0, ok1);
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    if (ok0[r]) {
+      mn = min(mn, x0[r]);
+      mx = max(mx, x0[r]);
+    }
+    if (ok1[r]) {
+      mn = min(mn, x1[r]);
+      mx = max(mx, x1[r]);
+    }
+  }
+  uint32_t base = 0;
+  const int fw = (flag & UTL_NOFOR) ? 0 : enc_for(x0, x1, ok0, ok1, g8_min(mn), g8_max(mx), n, base);
+  const uint32_t zz = (flag & UTL_DELTA) ? ((enc_delta(x0, x1, ok0, ok1) & kHZigZag) ? 1u : 0u) : 0u;
+  int bw, exc;
+  enc_width(x0, x1, (flag & UTL_NOPATCH) != 0, bw, exc);
+  const uint32_t m = low_mask(bw);
+  uint32_t eb = 0;  // StreamVByte data bytes of this thread's exceptions
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    eb += (ok0[r] && x0[r] > m) ? (uint32_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+    eb += (ok1[r] && x1[r] > m) ? (uint32_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+  }
+  eb = g8_add(eb);
+  const bool has_exc = exc > 0;
+  const int poff = kHdr + (has_exc ? kSvbLen : 0) + for_base_bytes(fw);
+  const int len = poff + 16 * bw + (has_exc ? exc_index_size(exc) + svb_ctrl_count(exc) + (int)eb : 0);
+  if (j == 0 && mine) {
+    out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
+    plan[b] = make_uint2((uint32_t)bw | ((uint32_t)fw << 8) | (zz << 10) | ((uint32_t)exc << 16), base);
+  }
+}
+
+// A block that would end past dst_cap (only an index that is not the scan
+// of the size pass's meta words can place it there) is not written.
+template <bool FULL>
+__device__ __forceinline__ void write_u32_wave(EncGrpLds* L, bool ok, int n, uint32_t flag, uint64_t voff,
+                                               const uint32_t* __restrict__ values, uint2 pl, uint64_t o,
+                                               uint8_t* __restrict__ dst, uint64_t dst_cap) {
+  const int blk = lane_id() >> 3, j = lane_id() & 7;
+  uint32_t x0[kRows], x1[kRows];
+  bool ok0[kRows], ok1[kRows];
+  enc_load(values + voff, n, j, FULL, x0, x1, ok0, ok1);
+  const int bw = (int)(pl.x & 0xFF), fw = (int)((pl.x >> 8) & 3), exc = (int)(pl.x >> 16);
+  const uint32_t base = pl.y;
+  uint32_t hflags = kTypeU32 | ((flag & UTL_SPECIAL) << 14) | ((uint32_t)fw << 15);
+  if (fw) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      x0[r] = ok0[r] ? x0[r] - base : 0u;
+      x1[r] = ok1[r] ? x1[r] - base : 0u;
+    }
+  }
+  if (flag & UTL_DELTA) {
+    // per-lane delta (delta.go:23-47), zigzag of every value as planned
+#pragma unroll
+    for (int r = kRows - 1; r >= 1; --r) {
+      x0[r] = ok0[r] ? x0[r] - x0[r - 1] : 0u;
+      x1[r] = ok1[r] ? x1[r] - x1[r - 1] : 0u;
+    }
+    hflags |= kHDelta;
+    if ((pl.x >> 10) & 1) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        x0[r] = zz_enc(x0[r]);
+        x1[r] = zz_enc(x1[r]);
+      }
+      hflags |= kHZigZag;
+    }
+  }
+  const bool has_exc = exc > 0;
+  Tail tail;
+  if (has_exc) tail_plan(L, blk, j, x0, x1, ok0, ok1, bw, exc, tail);
+  const int fbb = for_base_bytes(fw);
+  const int poff = kHdr + (has_exc ? kSvbLen : 0) + fbb;
+  const int svb_len = has_exc ? tail.svb_len : 0;
+  const uint64_t len = (uint64_t)(poff + 16 * bw + (has_exc ? exc_index_size(exc) + svb_len : 0));
+  if (!ok || o > dst_cap || len > dst_cap - o) return;
+  // header fields (bytes [0, poff)): header, svbLen, FOR base
+  const uint32_t hw = encode_header(n, bw, exc, hflags);
+  const uint32_t w1 = has_exc ? ((uint32_t)svb_len | (base << 16)) : base;
+  const uint32_t w2 = has_exc ? (fbb > 2 ? base >> 16 : 0u) : 0u;
+  const uint32_t hsel = j < 4 ? hw : w1;
+  if (j < poff) dst[o + j] = (uint8_t)(hsel >> (8 * (j & 3)));
+  if (j + 8 < poff) dst[o + j + 8] = (uint8_t)(w2 >> (8 * j));
+  const uint32_t m = low_mask(bw);
+  uint32_t p0[kRows], p1[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    p0[r] = x0[r] & m;
+    p1[r] = x1[r] & m;
+  }
+  pack_any_at(bw, p0, p1, dst, o + poff, j);
+  if (has_exc) tail_write(L, blk, j, exc, tail, dst, o + poff + 16 * bw);
+}
+
+// The job of lane group (lane >> 3) of wave `wave` of the workgroup: block
+// b (a lane without a block reads block b0's job, or the last block's when
+// the whole wave is past the end, so every read stays inside the tables).
+struct EncJob {
+  uint64_t b, voff;
+  uint32_t flag;
+  int n;
+  bool mine, ok, full;
+};
+
+__device__ __forceinline__ EncJob enc_job(const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_meta,
+                                          uint64_t nblocks, uint64_t b0) {
+  EncJob e;
+  e.mine = b0 + (lane_id() >> 3) < nblocks;
+  e.b = e.mine ? b0 + (lane_id() >> 3) : min(b0, nblocks - 1);
+  const uint32_t meta = in_meta[e.b];
+  e.flag = (meta >> 8) & 0xFF;
+  const int n_req = (int)(meta & 0xFF);
+  e.ok = e.mine && n_req >= 1 && n_req <= kBlock;  // else ErrInvalidBuffer (scalar.go:127-129)
+  e.n = e.ok ? n_req : 0;
+  e.voff = in_off[e.b];
+  e.full = __ballot(!e.ok || e.n != kBlock) == 0;
+  return e;
+}
+
+__global__ __launch_bounds__(256) void k_encode_u32_size(const uint32_t* __restrict__ values,
+                                                         const uint64_t* __restrict__ in_off,
+                                                         const uint32_t* __restrict__ in_meta, uint64_t nblocks,
+                                                         uint32_t* __restrict__ out_meta, uint2* __restrict__ plan) {
+  const uint64_t b0 = ((uint64_t)blockIdx.x * 4 + uniform(threadIdx.x >> 6)) * kWaveBlocksEnc;
+  if (b0 >= nblocks) return;  // (the whole wave)
+  const EncJob e = enc_job(in_off, in_meta, nblocks, b0);
+  if (e.full)
+    size_u32_wave<true>(e.b, e.mine, e.ok, e.n, e.flag, e.voff, values, out_meta, plan);
+  else
+    size_u32_wave<false>(e.b, e.mine, e.ok, e.n, e.flag, e.voff, values, out_meta, plan);
+}
+
+__global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u32_write(
+    const uint32_t* __restrict__ values, const uint64_t* __restrict__ in_off, const uint32_t* __restrict__ in_meta,
+    uint64_t nblocks, const uint2* __restrict__ plan, const uint64_t* __restrict__ block_off, uint8_t* __restrict__ dst,
+    uint64_t dst_cap) {
+  __shared__ EncGrpLds lds[kEncGrpWaves];
+  const uint32_t wave = uniform(threadIdx.x >> 6);
+  const uint64_t b0 = ((uint64_t)blockIdx.x * kEncGrpWaves + wave) * kWaveBlocksEnc;
+  if (b0 >= nblocks) return;  // (the whole wave; no workgroup barrier follows)
+  const EncJob e = enc_job(in_off, in_meta, nblocks, b0);
+  const uint2 pl = plan[e.b];
+  const uint64_t o = block_off[e.b];
+  if (e.full)
+    write_u32_wave<true>(&lds[wave], e.ok, e.n, e.flag, e.voff, values, pl, o, dst, dst_cap);
+  else
+    write_u32_wave<false>(&lds[wave], e.ok, e.n, e.flag, e.voff, values, pl, o, dst, dst_cap);
+}
+
 // ---- uint64 blocks, eight per wavefront ----
 
 template <int Ctrl>
@@ -1026,6 +1209,25 @@ hipError_t utl_launch_encode_u32_direct(const uint32_t* values, const uint64_t* 
   k_fill_lb<<<dim3((unsigned)((nlb + 255) / 256)), dim3(256), 0, stream>>>(lb, nlb);
   k_encode_u32<true><<<dim3((unsigned)ntiles), dim3(64 * kEncGrpWaves), 0, stream>>>(
       values, in_off, in_meta, nblocks, dst, 0, out_meta, work_out, lb, block_off, dst_cap);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_encode_u32_size(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                      uint64_t nblocks, uint32_t* out_meta, uint64_t* plan, hipStream_t stream) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t per_wg = 4 * (uint64_t)kWaveBlocksEnc;
+  k_encode_u32_size<<<dim3((unsigned)((nblocks + per_wg - 1) / per_wg)), dim3(256), 0, stream>>>(
+      values, in_off, in_meta, nblocks, out_meta, reinterpret_cast<uint2*>(plan));
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_encode_u32_write(const uint32_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                       uint64_t nblocks, const uint64_t* plan, const uint64_t* block_off, uint8_t* dst,
+                                       uint64_t dst_cap, hipStream_t stream) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  k_encode_u32_write<<<dim3((unsigned)((nblocks + per_wg - 1) / per_wg)), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, reinterpret_cast<const uint2*>(plan), block_off, dst, dst_cap);
   return hipGetLastError();
 }
 

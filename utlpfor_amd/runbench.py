@@ -139,12 +139,12 @@ def uneven(ctx: Context, nstreams: int = 200_000, reps: int = 5, seed: int = 3, 
 
 def encode_forms(ctx: Context, reps: int = 3, progress=None):
     """The batched encode of the 64 Mi delta postings (flag Delta) and of the
-    200,000 uneven streams, values resident in HBM, in both forms: slots +
-    index + compaction, and direct writes with look-back offsets. Yields one
-    record per workload: each form's median wall time of batch.encode over
-    reps runs after a warm-up (the whole call: job table, kernels,
-    allocations), input GB/s, equal bytes of the two forms, and the form the
-    tuner's cache would pick."""
+    200,000 uneven streams, values resident in HBM, in every form: slots +
+    index + compaction, direct writes with look-back offsets, and size pass
+    + index + write pass. Yields one record per workload: each form's median
+    wall time of batch.encode over reps runs after a warm-up (the whole
+    call: job table, kernels, allocations), input GB/s, equal bytes of all
+    forms, and the form the tuner's cache would pick."""
     say = progress or (lambda m: None)
     for name in ("delta64m", "uneven200k"):
         if name == "delta64m":
@@ -185,7 +185,7 @@ def encode_forms(ctx: Context, reps: int = 3, progress=None):
 
 def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblocks: int, reps: int = 3) -> dict:
     """GPU time of each encode form's kernels on buffers allocated once (HIP
-    events around job table + encode + index scan (+ compaction); the
+    events around job table + the form's kernels + index scan; the
     median of reps after a warm-up): what batch.encode costs without its
     allocations and host reads."""
     lib = _lib.load()
@@ -200,6 +200,7 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
     boff, ooff = DeviceArray(nblocks + 1, np.uint64, ctx), DeviceArray(nblocks + 1, np.uint64, ctx)
     dst = DeviceArray(nblocks * batch.MAX_BLOCK32 + 64, np.uint8, ctx)
     slots = DeviceArray(nblocks * batch.SLOT32, np.uint8, ctx)
+    plan = DeviceArray(nblocks, np.uint64, ctx)
     nb_out = C.c_size_t()
 
     def jobs():
@@ -219,9 +220,17 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
                                         meta.ptr, boff.ptr, C.byref(nb_out)), "direct")
         check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
 
+    def sized():
+        jobs()
+        check(lib.utl_encode_u32_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr, plan.ptr),
+              "size")
+        check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
+        check(lib.utl_encode_u32_write(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, plan.ptr, boff.ptr,
+                                       dst.ptr, dst.nbytes), "write")
+
     out = {}
     try:
-        for m, fn in (("slot", slot), ("direct", direct)):
+        for m, fn in (("slot", slot), ("direct", direct), ("sized", sized)):
             fn()
             ts = []
             for _ in range(reps):
@@ -233,7 +242,7 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
                 ts.append(e0.elapsed_ms(e1))
             out[m] = round(float(np.median(ts)), 4)
     finally:
-        for a in (d_val, d_blk, d_fl, in_off, in_meta, meta, boff, ooff, dst, slots):
+        for a in (d_val, d_blk, d_fl, in_off, in_meta, meta, boff, ooff, dst, slots, plan):
             a.free()
     return out
 
