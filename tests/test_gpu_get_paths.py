@@ -96,7 +96,7 @@ def test_dir_get_equals_search_and_values(ctx, is64):
     good = ok & (s_dir == 0)
     assert good.sum() >= 0.5 * ok.sum()
     assert np.array_equal(g_dir[good], v[pos[good]])
-    # the Python wrapper takes the directory by default
+    # the Python wrapper takes the directory for a store of many streams
     vals, gst = batch.get_batch(ctx, s, batch.DeviceArray.from_numpy(pos, ctx))
     ctx.sync()
     assert np.array_equal(vals.to_numpy(), g_dir) and np.array_equal(gst.to_numpy(), s_dir)

@@ -254,16 +254,22 @@ def decode_meta(ctx: Context, s: DeviceStream, wave: DeviceArray, out: DeviceArr
 
 
 def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: DeviceArray | None = None,
-              status: DeviceArray | None = None, blocks: DeviceArray | None = None, use_dir: bool = True):
+              status: DeviceArray | None = None, blocks: DeviceArray | None = None, use_dir: bool | None = None):
     """Batched Get of stream positions (or of (block, in-block pos) pairs).
     Stream positions go through the stream's position directory
-    (utl_get_batch_dir) unless use_dir is False (utl_get_batch's search)."""
+    (utl_get_batch_dir) when its block numbers drift from position / 128
+    (more than 16 blocks beyond ceil(values / 128): partial blocks inside
+    the stream, as in a store of many streams), else through utl_get_batch's
+    search from position / 128, which is exact for a stream of full blocks
+    and saves the directory read. use_dir True / False forces either."""
     lib = _lib.load()
     n = positions.n
     if out is None:
         out = DeviceArray(max(n, 1), np.uint64 if s.is64 else np.uint32, ctx)
     if status is None:
         status = DeviceArray(max(n, 1), np.int32, ctx)
+    if use_dir is None:
+        use_dir = s.nblocks > (s.nvalues + 127) // 128 + 16
     if blocks is None and use_dir and s.nblocks > 0:
         d = s.ensure_pos_dir()
         check(lib.utl_get_batch_dir(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
