@@ -331,6 +331,18 @@ int utl_encode_u32_size(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *
 int utl_encode_u32_write(utl_ctx *ctx, const uint32_t *d_values, const uint64_t *d_in_off,
                          const uint32_t *d_in_meta, size_t nblocks, const uint64_t *d_plan,
                          const uint64_t *d_block_off, uint8_t *d_dst, size_t dst_cap);
+/* Batched PackUint64 (uint64.go:279) in two passes with the index scan
+ * between them: utl_encode_u64_size writes d_out_meta (as
+ * utl_encode_blocks does); after utl_index_build over it,
+ * utl_encode_u64_write encodes each block again (packUint64Scalar,
+ * uint64.go:302-333) and writes it at d_block_off[b] of d_dst: the same
+ * bytes as utl_encode_blocks + utl_compact, with no slots. A sub-block that
+ * would end past dst_cap is not written. */
+int utl_encode_u64_size(utl_ctx *ctx, const uint64_t *d_values, const uint64_t *d_in_off,
+                        const uint32_t *d_in_meta, size_t nblocks, uint32_t *d_out_meta);
+int utl_encode_u64_write(utl_ctx *ctx, const uint64_t *d_values, const uint64_t *d_in_off,
+                         const uint32_t *d_in_meta, size_t nblocks, const uint64_t *d_block_off, uint8_t *d_dst,
+                         size_t dst_cap);
 /* Concatenates encoded slots into d_dst at d_block_off (from utl_index_build
  * over d_out_meta): byte-identical to appending the blocks with Append.
  * Writes exactly the stream's d_block_off[nblocks] bytes. */

@@ -87,13 +87,14 @@ def test_uneven_20000_streams_encode_bytes(ctx, orc, method):
     s.free()
 
 
+@pytest.mark.parametrize("method", ["slot", "sized"])
 @pytest.mark.parametrize("which", ["timestamps64", "uniform64"])
-def test_u64_encode_bytes_and_every_block(ctx, orc, which):
+def test_u64_encode_bytes_and_every_block(ctx, orc, which, method):
     from utlpfor_amd import batch, workloads
     v = workloads.timestamps64(32 << 20) if which == "timestamps64" else workloads.uniform64(8 << 20)
-    say(f"{which}: {v.size} values; oracle pack")
+    say(f"{which}: {v.size} values; oracle pack; GPU encode ({method})")
     want, bl, bc = orc.pack_stream(0, v, nthreads=16)
-    s = batch.encode(ctx, v, flag=0)
+    s = batch.encode(ctx, v, flag=0, method=method)
     assert np.array_equal(s.meta.to_numpy(), _meta_of(bl, bc))
     assert np.array_equal(s.bytes_host(), want)
     say("decode of every block vs the oracle")

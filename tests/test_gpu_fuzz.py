@@ -267,8 +267,8 @@ def _edge_values(rng, n, is64):
 @pytest.mark.parametrize("is64", [False, True])
 @pytest.mark.parametrize("method", ["slot", "direct", "sized"])
 def test_fuzz_encode_bytes(ctx, orc, seed, is64, method):
-    """GPU encode of many short streams with every flag combination, in both
-    encode forms (uint64 has the slot form only): the packed bytes equal the
+    """GPU encode of many short streams with every flag combination, in every
+    encode form (uint64: the slot and sized forms): the packed bytes equal the
     oracle's PackUint32/64(flag|Append) stream by stream, and the meta words
     give its block lengths and counts."""
     from utlpfor_amd import batch

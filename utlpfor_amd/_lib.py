@@ -64,6 +64,8 @@ SIGNATURES = {
     "utl_compact": (C.c_int, [vp, C.c_int, vp, vp, sz, vp]),
     "utl_encode_u32_size": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),
     "utl_encode_u32_write": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, vp, sz]),
+    "utl_encode_u64_size": (C.c_int, [vp, vp, vp, vp, sz, vp]),
+    "utl_encode_u64_write": (C.c_int, [vp, vp, vp, vp, sz, vp, vp, sz]),
     "utl_encode_u32_direct": (C.c_int, [vp, vp, vp, vp, sz, vp, sz, vp, vp, C.POINTER(sz)]),
     "utl_make_jobs": (C.c_int, [vp, vp, vp, vp, sz, vp, vp]),
     "utl_graph_decode_create": (vp, [vp, C.c_int, C.c_int, vp, vp, sz, vp, vp, vp, vp]),

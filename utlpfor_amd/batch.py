@@ -19,6 +19,7 @@ from .device import Context, DeviceArray, check
 
 SLOT32, SLOT64 = 1024, 2048
 MAX_BLOCK32 = 1018  # MaxBlockLength32(0), the longest uint32 block (scalar.go:336-357)
+MAX_BLOCK64 = 2038  # MaxBlockLength64(0), the longest uint64 block (uint64.go:451-453)
 
 
 @dataclass
@@ -77,15 +78,13 @@ def encode_method(ctx: Context, is64: bool, nblocks: int, nvalues: int, method: 
     """The encode form to use: "slot" (each block into a fixed slot, then the
     index scan and a compaction pass), "direct" (uint32 only: blocks
     written straight to their place, offsets by look-back) or "sized"
-    (uint32 only: a size pass, the index scan, a write pass at the
-    offsets). "auto" takes the autotuner's cached choice for this shape of
-    work, else the sized form for uint32 and the slot form for uint64."""
-    if is64:
-        return "slot"
+    (a size pass, the index scan, a write pass at the offsets). "auto"
+    takes the autotuner's cached choice for this shape of uint32 work, else
+    the sized form."""
     if method in ENCODE_METHODS:
-        return method
+        return "slot" if is64 and method == "direct" else method
     from . import autotune
-    return autotune.cached_encode_method(ctx, nblocks, nvalues) or "sized"
+    return (None if is64 else autotune.cached_encode_method(ctx, nblocks, nvalues)) or "sized"
 
 
 def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flags=None,
@@ -115,7 +114,17 @@ def encode(ctx: Context, values, flag: int = 0, stream_lengths=None, stream_flag
                             in_meta.ptr), "utl_make_jobs")
     meta = DeviceArray(nblocks, np.uint32, ctx)
     form = encode_method(ctx, bool(is64), nblocks, n, method)
-    if form == "sized":
+    if form == "sized" and is64:
+        # lengths, the index scan, then every block encoded again at its place
+        check(lib.utl_encode_u64_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr),
+              "utl_encode_u64_size")
+        boff, ooff = build_index(ctx, meta, nblocks)
+        total = int(boff.read(nblocks, 1)[0]) if nblocks else 0
+        data = DeviceArray(total + 64, np.uint8, ctx)
+        check(lib.utl_encode_u64_write(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, boff.ptr, data.ptr,
+                                       data.nbytes), "utl_encode_u64_write")
+        scratch = ()
+    elif form == "sized":
         # decisions and lengths, the index scan, then every block at its place
         plan = DeviceArray(max(nblocks, 1), np.uint64, ctx)
         check(lib.utl_encode_u32_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr, plan.ptr),

@@ -723,6 +723,26 @@ int utl_encode_u32_write(utl_ctx* ctx, const uint32_t* d_values, const uint64_t*
   return UTL_OK;
 }
 
+int utl_encode_u64_size(utl_ctx* ctx, const uint64_t* d_values, const uint64_t* d_in_off, const uint32_t* d_in_meta,
+                        size_t nblocks, uint32_t* d_out_meta) {
+  if (nblocks && (!d_values || !d_in_off || !d_in_meta || !d_out_meta)) return UTL_ERR_ARGUMENT;
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_encode_u64_size(d_values, d_in_off, d_in_meta, nblocks, d_out_meta, c->stream));
+  return UTL_OK;
+}
+
+int utl_encode_u64_write(utl_ctx* ctx, const uint64_t* d_values, const uint64_t* d_in_off, const uint32_t* d_in_meta,
+                         size_t nblocks, const uint64_t* d_block_off, uint8_t* d_dst, size_t dst_cap) {
+  if (nblocks && (!d_values || !d_in_off || !d_in_meta || !d_block_off || !d_dst)) return UTL_ERR_ARGUMENT;
+  int err;
+  utl_ctx* c = resolve(ctx, &err);
+  if (!c) return err;
+  HIP_TRY(utl_launch_encode_u64_write(d_values, d_in_off, d_in_meta, nblocks, d_block_off, d_dst, dst_cap, c->stream));
+  return UTL_OK;
+}
+
 int utl_compact(utl_ctx* ctx, int is_u64, const uint8_t* d_slots, const uint64_t* d_block_off, size_t nblocks,
                 uint8_t* d_dst) {
   if (nblocks && (!d_slots || !d_block_off || !d_dst)) return UTL_ERR_ARGUMENT;

@@ -33,6 +33,15 @@ hipError_t utl_launch_encode_u32_write(const uint32_t* values, const uint64_t* i
 hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
                                  uint64_t nblocks, uint8_t* slots, uint32_t slot_bytes, uint32_t* out_meta,
                                  hipStream_t stream);
+// Sized form of the u64 encode: the size pass writes out_meta only; after
+// the caller's index scan of out_meta, the write pass encodes block b again
+// and writes it at block_off[b] of dst (a sub-block ending past dst_cap is
+// not written). Byte-identical to the slot form + compaction.
+hipError_t utl_launch_encode_u64_size(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                      uint64_t nblocks, uint32_t* out_meta, hipStream_t stream);
+hipError_t utl_launch_encode_u64_write(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                       uint64_t nblocks, const uint64_t* block_off, uint8_t* dst, uint64_t dst_cap,
+                                       hipStream_t stream);
 // Concatenation of encoded slots: tiles (the uint32 layout) or one block per
 // slot (uint64), into dst at the block offsets boff.
 hipError_t utl_launch_compact(const uint8_t* slots, uint32_t slot_bytes, bool tiles, const uint64_t* boff,

@@ -878,7 +878,7 @@ struct SubHead {
   uint32_t hw, svb_len, b2len;
   uint64_t base;
   int fbb, poff;
-  bool has_exc;
+  bool has_exc, wrote;
 };
 
 __device__ __forceinline__ uint32_t head_byte(const SubHead& h, int k) {
@@ -900,13 +900,15 @@ __device__ __forceinline__ void put_head_at(const SubHead& h, uint8_t* __restric
 
 // One 32-bit sub-block of a uint64 block (packBlockScalar's steps, as for a
 // uint32 block) with header type bits tflags: its payload and exception
-// section written at byte o of g when `write`; its header fields returned
-// (written by the caller, which knows block2Len last). for64: the values are
-// already minus the FOR64 base `base64` and the sub-block carries it.
+// section written at byte o of g when `write` and the sub-block ends at or
+// before byte `limit` (h.wrote); its header fields returned (written by the
+// caller, which knows block2Len last). for64: the values are already minus
+// the FOR64 base `base64` and the sub-block carries it.
 __device__ __forceinline__ SubHead enc_sub_u64(EncGrpLds* L, int blk, int j, uint32_t* x0, uint32_t* x1,
                                                const bool* ok0, const bool* ok1, int n, uint32_t flag,
                                                uint32_t tflags, bool for64, uint64_t base64, bool combine,
-                                               uint8_t* __restrict__ g, uint64_t o, bool write) {
+                                               uint8_t* __restrict__ g, uint64_t o, bool write,
+                                               uint64_t limit) {
   uint32_t hflags = tflags | ((flag & UTL_SPECIAL) << 14);
   int fw = 0;
   uint32_t base = 0;
@@ -939,7 +941,9 @@ __device__ __forceinline__ SubHead enc_sub_u64(EncGrpLds* L, int blk, int j, uin
   Tail tail;
   if (h.has_exc) tail_plan(L, blk, j, x0, x1, ok0, ok1, bw, exc, tail);
   h.svb_len = h.has_exc ? (uint32_t)tail.svb_len : 0u;
-  if (write) {
+  const uint64_t len = (uint64_t)(h.poff + 16 * bw + (h.has_exc ? exc_index_size(exc) + (int)h.svb_len : 0));
+  h.wrote = write && o <= limit && len <= limit - o;
+  if (h.wrote) {
     const uint32_t m = low_mask(bw);
     uint32_t p0[kRows], p1[kRows];
 #pragma unroll
@@ -958,18 +962,25 @@ __device__ __forceinline__ int sub_len(const SubHead& h) {
   return h.poff + 16 * bw + (h.has_exc ? exc_index_size(exc) + (int)h.svb_len : 0);
 }
 
-// Encodes the blocks of a u64 job table into slots, eight per wavefront,
-// following packUint64Scalar (uint64.go:302-333): FOR64 when the minimum is
-// above 0 and max - min < 2^32 (one sub-block of value - min, flag NoFOR,
-// the 8-byte base); fit-32 when every value is below 2^32 (one sub-block);
-// else combine: the low halves, then the high halves, each a sub-block with
-// the block's flag, the first carrying the second's length. Block b starts
-// at byte 0 of slot b.
+// Encodes the blocks of a u64 job table, eight per wavefront, following
+// packUint64Scalar (uint64.go:302-333): FOR64 when the minimum is above 0
+// and max - min < 2^32 (one sub-block of value - min, flag NoFOR, the 8-byte
+// base); fit-32 when every value is below 2^32 (one sub-block); else
+// combine: the low halves, then the high halves, each a sub-block with the
+// block's flag, the first carrying the second's length. MODE 0: block b
+// from byte 0 of slot b of dst; MODE 1: lengths only (out_meta); MODE 2:
+// block b at block_off[b] of dst (dst_cap bytes; the scan of MODE 1's meta
+// words), a sub-block that would end past dst_cap not written.
+constexpr int kEncSlots = 0, kEncSize = 1, kEncWrite = 2;
+
+template <int MODE>
 __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u64(const uint64_t* __restrict__ values,
                                                                   const uint64_t* __restrict__ in_off,
                                                                   const uint32_t* __restrict__ in_meta,
-                                                                  uint64_t nblocks, uint8_t* __restrict__ slots,
-                                                                  uint32_t slot_bytes, uint32_t* __restrict__ out_meta) {
+                                                                  uint64_t nblocks, uint8_t* __restrict__ dst,
+                                                                  uint32_t slot_bytes,
+                                                                  const uint64_t* __restrict__ block_off,
+                                                                  uint64_t dst_cap, uint32_t* __restrict__ out_meta) {
   __shared__ EncGrpLds lds[kEncGrpWaves];
   const uint32_t wave = uniform(threadIdx.x >> 6);
   const uint64_t b0 = ((uint64_t)blockIdx.x * kEncGrpWaves + wave) * kWaveBlocksEnc;
@@ -1011,9 +1022,11 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u64(const uint64_t
     x0[r] = ok0[r] ? (uint32_t)(for64 ? a0[r] - mn : a0[r]) : 0u;
     x1[r] = ok1[r] ? (uint32_t)(for64 ? a1[r] - mn : a1[r]) : 0u;
   }
-  const uint64_t o = b * slot_bytes;
+  const uint64_t o = MODE == kEncSlots ? b * slot_bytes : MODE == kEncWrite ? block_off[b] : 0;
+  const bool wr = MODE != kEncSize;
+  const uint64_t limit = MODE == kEncWrite ? dst_cap : ~0ull;
   SubHead h1 = enc_sub_u64(L, blk, j, x0, x1, ok0, ok1, n, for64 ? flag | UTL_NOFOR : flag,
-                           kTypeU64 | (combine ? kHCombine : 0u), for64, mn, combine, slots, o, ok);
+                           kTypeU64 | (combine ? kHCombine : 0u), for64, mn, combine, dst, o, wr && ok, limit);
   int len = sub_len(h1);
   if (combine && ok) {  // the groups of combine blocks: the high halves after the first sub-block
 #pragma unroll
@@ -1022,14 +1035,14 @@ __global__ __launch_bounds__(64 * kEncGrpWaves) void k_encode_u64(const uint64_t
       x1[r] = (uint32_t)(a1[r] >> 32);
     }
     lds_fence();  // the first sub-block's exception bytes are written out
-    const SubHead h2 = enc_sub_u64(L, blk, j, x0, x1, ok0, ok1, n, flag, kTypeU64, false, 0, false, slots,
-                                   o + len, true);
-    put_head_at(h2, slots, o + len, j);
+    const SubHead h2 = enc_sub_u64(L, blk, j, x0, x1, ok0, ok1, n, flag, kTypeU64, false, 0, false, dst, o + len,
+                                   wr, limit);
+    if (h2.wrote) put_head_at(h2, dst, o + len, j);
     h1.b2len = (uint32_t)sub_len(h2);
     len += (int)h1.b2len;
   }
-  if (ok) put_head_at(h1, slots, o, j);
-  if (j == 0 && mine) out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
+  if (h1.wrote) put_head_at(h1, dst, o, j);
+  if (MODE != kEncWrite && j == 0 && mine) out_meta[b] = ok ? ((uint32_t)len | ((uint32_t)n << 24)) : 0xFFFFFFFFu;
 }
 
 // Concatenates slots of one block each (the uint64 layout) into dst (block
@@ -1237,8 +1250,27 @@ hipError_t utl_launch_encode_u64(const uint64_t* values, const uint64_t* in_off,
   if (nblocks == 0) return hipSuccess;
   const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
   const uint64_t grid = (nblocks + per_wg - 1) / per_wg;
-  k_encode_u64<<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(values, in_off, in_meta, nblocks, slots,
-                                                                               slot_bytes, out_meta);
+  k_encode_u64<kEncSlots><<<dim3((unsigned)grid), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, slots, slot_bytes, nullptr, 0, out_meta);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_encode_u64_size(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                      uint64_t nblocks, uint32_t* out_meta, hipStream_t stream) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  k_encode_u64<kEncSize><<<dim3((unsigned)((nblocks + per_wg - 1) / per_wg)), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, nullptr, 0, nullptr, 0, out_meta);
+  return hipGetLastError();
+}
+
+hipError_t utl_launch_encode_u64_write(const uint64_t* values, const uint64_t* in_off, const uint32_t* in_meta,
+                                       uint64_t nblocks, const uint64_t* block_off, uint8_t* dst, uint64_t dst_cap,
+                                       hipStream_t stream) {
+  if (nblocks == 0) return hipSuccess;
+  const uint64_t per_wg = (uint64_t)kEncGrpWaves * kWaveBlocksEnc;
+  k_encode_u64<kEncWrite><<<dim3((unsigned)((nblocks + per_wg - 1) / per_wg)), dim3(64 * kEncGrpWaves), 0, stream>>>(
+      values, in_off, in_meta, nblocks, dst, 0, block_off, dst_cap, nullptr);
   return hipGetLastError();
 }
 

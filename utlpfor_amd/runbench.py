@@ -183,7 +183,8 @@ def encode_forms(ctx: Context, reps: int = 3, progress=None):
         yield rec
 
 
-def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblocks: int, reps: int = 3) -> dict:
+def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblocks: int, reps: int = 3,
+                     is64: bool = False) -> dict:
     """GPU time of each encode form's kernels on buffers allocated once (HIP
     events around job table + the form's kernels + index scan; the
     median of reps after a warm-up): what batch.encode costs without its
@@ -198,8 +199,8 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
     in_off, in_meta = DeviceArray(nblocks, np.uint64, ctx), DeviceArray(nblocks, np.uint32, ctx)
     meta = DeviceArray(nblocks, np.uint32, ctx)
     boff, ooff = DeviceArray(nblocks + 1, np.uint64, ctx), DeviceArray(nblocks + 1, np.uint64, ctx)
-    dst = DeviceArray(nblocks * batch.MAX_BLOCK32 + 64, np.uint8, ctx)
-    slots = DeviceArray(nblocks * batch.SLOT32, np.uint8, ctx)
+    dst = DeviceArray(nblocks * (batch.MAX_BLOCK64 if is64 else batch.MAX_BLOCK32) + 64, np.uint8, ctx)
+    slots = DeviceArray(nblocks * (batch.SLOT64 if is64 else batch.SLOT32), np.uint8, ctx)
     plan = DeviceArray(nblocks, np.uint64, ctx)
     nb_out = C.c_size_t()
 
@@ -209,10 +210,10 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
 
     def slot():
         jobs()
-        check(lib.utl_encode_blocks(ctx.handle, 0, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr, meta.ptr),
-              "encode")
+        check(lib.utl_encode_blocks(ctx.handle, int(is64), dv.ptr, in_off.ptr, in_meta.ptr, nblocks, slots.ptr,
+                                    meta.ptr), "encode")
         check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
-        check(lib.utl_compact(ctx.handle, 0, slots.ptr, boff.ptr, nblocks, dst.ptr), "compact")
+        check(lib.utl_compact(ctx.handle, int(is64), slots.ptr, boff.ptr, nblocks, dst.ptr), "compact")
 
     def direct():
         jobs()
@@ -222,6 +223,12 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
 
     def sized():
         jobs()
+        if is64:
+            check(lib.utl_encode_u64_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr), "size")
+            check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
+            check(lib.utl_encode_u64_write(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, boff.ptr, dst.ptr,
+                                           dst.nbytes), "write")
+            return
         check(lib.utl_encode_u32_size(ctx.handle, dv.ptr, in_off.ptr, in_meta.ptr, nblocks, meta.ptr, plan.ptr),
               "size")
         check(lib.utl_index_build(ctx.handle, meta.ptr, nblocks, boff.ptr, ooff.ptr), "index")
@@ -230,7 +237,8 @@ def encode_device_ms(ctx: Context, dv: DeviceArray, lens, flags, flag: int, nblo
 
     out = {}
     try:
-        for m, fn in (("slot", slot), ("direct", direct), ("sized", sized)):
+        forms = (("slot", slot), ("sized", sized)) if is64 else (("slot", slot), ("direct", direct), ("sized", sized))
+        for m, fn in forms:
             fn()
             ts = []
             for _ in range(reps):
@@ -407,6 +415,9 @@ def u64_and_get(ctx: Context, reps: int = 10, n_ts: int = 32 << 20, n_uni: int =
             a, b = int(ooff[k]), int(ooff[k + 1])
             exact &= bool(np.array_equal(got[a:b], v[a:b]))
         rec = _decode_record(name, s, ms, v.size, 8, {"blocks_ok": int(okb.sum()), "bit_exact_sampled": exact})
+        dv = DeviceArray.from_numpy(v, ctx)
+        rec["encode_device_ms"] = encode_device_ms(ctx, dv, None, None, 0, s.nblocks, reps=3, is64=True)
+        dv.free()
         if "timestamps" in name:
             rng = np.random.default_rng(7)
             pos = rng.integers(0, v.size, n_get).astype(np.int64)
