@@ -287,8 +287,8 @@ class Engine:
             return r.positions.size * (8 + 8 + 8 + 4) + 4 * _ALIGN
         v = r.values
         nb = (v.size + 127) // 128
-        slot = batch.SLOT64 if v.dtype == np.uint64 else batch.SLOT32
-        return v.nbytes + nb * (2 * slot + 8 + 4 + 4 + 16) + 64 + 12 * _ALIGN
+        longest = batch.MAX_BLOCK64 if v.dtype == np.uint64 else batch.MAX_BLOCK32
+        return v.nbytes + nb * (longest + 8 + 8 + 4 + 4 + 16) + 64 + 12 * _ALIGN
 
     def check_request(self, r: Request, limit: int | None = None) -> None:
         """Rejects a request outside its stream, or larger than the device
@@ -428,16 +428,24 @@ class Engine:
         in_meta = self.heap.take(4 * nb)
         check(lib.utl_make_jobs(ctx.handle, d_val_off, d_blk_off, d_flags, len(reqs), in_off, in_meta),
               "utl_make_jobs")
-        slot = batch.SLOT64 if is64 else batch.SLOT32
-        slots = self.heap.take(nb * slot)
+        # the sized form: lengths (and for uint32 each block's plan), the
+        # index scan, then every block written at its offset
         meta = self.heap.take(4 * nb)
-        check(lib.utl_encode_blocks(ctx.handle, int(is64), d_vals, in_off, in_meta, nb, slots, meta),
-              "utl_encode_blocks")
         boff = self.heap.take(8 * (nb + 1))
         ooff = self.heap.take(8 * (nb + 1))
-        check(lib.utl_index_build(ctx.handle, meta, nb, boff, ooff), "utl_index_build")
-        data = self.heap.take(nb * slot + 64)
-        check(lib.utl_compact(ctx.handle, int(is64), slots, boff, nb, data), "utl_compact")
+        cap = nb * (batch.MAX_BLOCK64 if is64 else batch.MAX_BLOCK32) + 64
+        data = self.heap.take(cap)
+        if is64:
+            check(lib.utl_encode_u64_size(ctx.handle, d_vals, in_off, in_meta, nb, meta), "utl_encode_u64_size")
+            check(lib.utl_index_build(ctx.handle, meta, nb, boff, ooff), "utl_index_build")
+            check(lib.utl_encode_u64_write(ctx.handle, d_vals, in_off, in_meta, nb, boff, data, cap),
+                  "utl_encode_u64_write")
+        else:
+            plan = self.heap.take(8 * nb)
+            check(lib.utl_encode_u32_size(ctx.handle, d_vals, in_off, in_meta, nb, meta, plan), "utl_encode_u32_size")
+            check(lib.utl_index_build(ctx.handle, meta, nb, boff, ooff), "utl_index_build")
+            check(lib.utl_encode_u32_write(ctx.handle, d_vals, in_off, in_meta, nb, plan, boff, data, cap),
+                  "utl_encode_u32_write")
 
         def deliver():
             bo = self._d2h(boff, nb + 1, np.uint64)
