@@ -154,6 +154,7 @@ class _HostStage:
         self.ctx, self.lib = ctx, _lib.load()
         self.pin = PinnedBuffer(nbytes)
         self.pool = ThreadPoolExecutor(max_workers=threads)
+        self.uploader = ThreadPoolExecutor(max_workers=1)
         self.piece = 4 << 20  # bytes one thread copies at a time
 
     def fetch(self, dev: int, nbytes: int, dsts: list[tuple[int, np.ndarray]]) -> None:
@@ -188,34 +189,54 @@ class _HostStage:
     def put(self, dev: int, nbytes: int, srcs: list[tuple[int, np.ndarray]]) -> None:
         """The other direction: array a's bytes to device [dev + off, ...)
         for every (off, a) in srcs (contiguous arrays, disjoint ranges that
-        cover [0, nbytes)), through the pinned buffer chunk by chunk."""
+        cover [0, nbytes)), through the two halves of the pinned buffer in
+        turn: the host threads fill one half while the other half's upload
+        runs (on its own thread; the upload call waits for its copy)."""
         srcs = sorted(((off, a.reshape(-1).view(np.uint8)) for off, a in srcs if a.nbytes), key=lambda t: t[0])
-        cap = self.pin.nbytes
-        c0, k = 0, 0
-        while c0 < nbytes:
-            c1 = min(c0 + cap, nbytes)
-            jobs = []
-            j = k
-            while j < len(srcs) and srcs[j][0] < c1:
-                off, a = srcs[j]
-                lo, hi = max(off, c0), min(off + a.nbytes, c1)
-                for x in range(lo, hi, self.piece):
-                    y = min(x + self.piece, hi)
-                    dst, src = self.pin.bytes[x - c0:y - c0], a[x - off:y - off]
-                    if hi - lo <= self.piece:
-                        dst[:] = src
-                    else:
-                        jobs.append(self.pool.submit(np.copyto, dst, src))
-                if off + a.nbytes <= c1:
-                    k = j + 1
-                j += 1
-            for f in jobs:
+        cap = max(1, self.pin.nbytes // 2)
+        c0, k, n = 0, 0, 0
+        uploads = [None, None]  # the upload under way from each half
+        try:
+            while c0 < nbytes:
+                c1 = min(c0 + cap, nbytes)
+                h = n & 1
+                if uploads[h] is not None:
+                    uploads[h].result()  # the half is free again
+                    uploads[h] = None
+                base = h * cap
+                jobs = []
+                j = k
+                while j < len(srcs) and srcs[j][0] < c1:
+                    off, a = srcs[j]
+                    lo, hi = max(off, c0), min(off + a.nbytes, c1)
+                    for x in range(lo, hi, self.piece):
+                        y = min(x + self.piece, hi)
+                        dst, src = self.pin.bytes[base + x - c0:base + y - c0], a[x - off:y - off]
+                        if hi - lo <= self.piece:
+                            dst[:] = src
+                        else:
+                            jobs.append(self.pool.submit(np.copyto, dst, src))
+                    if off + a.nbytes <= c1:
+                        k = j + 1
+                    j += 1
+                for f in jobs:
+                    f.result()
+                uploads[h] = self.uploader.submit(self._upload, dev + c0, self.pin.ptr + base, c1 - c0)
+                c0, n = c1, n + 1
+        finally:
+            for f in uploads:
+                if f is not None:
+                    f.exception()  # every upload ends before the buffer is reused or the caller goes on
+        for f in uploads:
+            if f is not None:
                 f.result()
-            check(self.lib.utl_memcpy_h2d(self.ctx.handle, dev + c0, self.pin.ptr, c1 - c0), "h2d")
-            c0 = c1
+
+    def _upload(self, dev: int, host: int, nbytes: int) -> None:
+        check(self.lib.utl_memcpy_h2d(self.ctx.handle, dev, host, nbytes), "h2d")
 
     def close(self) -> None:
         self.pool.shutdown()
+        self.uploader.shutdown()
         self.pin.free()
 
 
