@@ -145,46 +145,70 @@ class _Heap:
 
 
 class _HostStage:
-    """Delivery of results to the host. A device range comes over in chunks
-    of a pinned buffer (full link rate), and each chunk is copied out to the
-    result arrays it holds by a pool of threads (numpy copies release the
-    GIL; one host thread copies at about 14 GB/s)."""
+    """Copies between the device and request arrays in host memory through a
+    pinned buffer (full link rate) in chunks, each copied out to (or in
+    from) the arrays it holds by a pool of threads (numpy copies release the
+    GIL; one host thread copies at about 14 GB/s). The buffer's two halves
+    take chunks in turn, so the link's copy of one chunk overlaps the host
+    copies of the next."""
 
     def __init__(self, ctx: Context, nbytes: int = 256 << 20, threads: int = 8):
         self.ctx, self.lib = ctx, _lib.load()
         self.pin = PinnedBuffer(nbytes)
         self.pool = ThreadPoolExecutor(max_workers=threads)
-        self.uploader = ThreadPoolExecutor(max_workers=1)
+        self.mover = ThreadPoolExecutor(max_workers=1)  # the link's copies, one at a time
         self.piece = 4 << 20  # bytes one thread copies at a time
 
     def fetch(self, dev: int, nbytes: int, dsts: list[tuple[int, np.ndarray]]) -> None:
         """Copies device bytes [dev + off, dev + off + a.nbytes) into array a
         for every (off, a) in dsts (contiguous arrays, disjoint ranges inside
-        [0, nbytes))."""
+        [0, nbytes)), through the two halves of the pinned buffer in turn:
+        the next chunk comes over into one half (on its own thread) while
+        the host threads copy the current one out of the other."""
         dsts = sorted(((off, a.reshape(-1).view(np.uint8)) for off, a in dsts if a.nbytes), key=lambda t: t[0])
-        cap = self.pin.nbytes
-        c0, k = 0, 0
-        while c0 < nbytes and k < len(dsts):
-            c1 = min(c0 + cap, nbytes)
-            check(self.lib.utl_memcpy_d2h(self.ctx.handle, self.pin.ptr, dev + c0, c1 - c0), "d2h")
-            jobs = []
-            j = k
-            while j < len(dsts) and dsts[j][0] < c1:
-                off, a = dsts[j]
-                lo, hi = max(off, c0), min(off + a.nbytes, c1)
-                for x in range(lo, hi, self.piece):
-                    y = min(x + self.piece, hi)
-                    src, dst = self.pin.bytes[x - c0:y - c0], a[x - off:y - off]
-                    if hi - lo <= self.piece:
-                        dst[:] = src
-                    else:
-                        jobs.append(self.pool.submit(np.copyto, dst, src))
-                if off + a.nbytes <= c1:
-                    k = j + 1
-                j += 1
-            for f in jobs:
-                f.result()
-            c0 = c1
+        if not dsts or nbytes <= 0:
+            return
+        cap = max(1, self.pin.nbytes // 2)
+        end = min(nbytes, max(off + a.nbytes for off, a in dsts))  # nothing past the last array comes over
+        chunks = [(c0, min(c0 + cap, end)) for c0 in range(0, end, cap)]
+        k = 0
+        nxt = None
+        if len(chunks) == 1:  # one chunk: nothing to overlap, no thread hop
+            self._download(self.pin.ptr, dev, chunks[0][1])
+        else:
+            nxt = self.mover.submit(self._download, self.pin.ptr, dev, chunks[0][1])
+        try:
+            for n, (c0, c1) in enumerate(chunks):
+                if nxt is not None:
+                    nxt.result()
+                nxt = None
+                base = (n & 1) * cap
+                if n + 1 < len(chunks):  # the other half's copies out ended in the last round
+                    b0, b1 = chunks[n + 1]
+                    nxt = self.mover.submit(self._download, self.pin.ptr + ((n + 1) & 1) * cap, dev + b0, b1 - b0)
+                jobs = []
+                j = k
+                while j < len(dsts) and dsts[j][0] < c1:
+                    off, a = dsts[j]
+                    lo, hi = max(off, c0), min(off + a.nbytes, c1)
+                    for x in range(lo, hi, self.piece):
+                        y = min(x + self.piece, hi)
+                        src, dst = self.pin.bytes[base + x - c0:base + y - c0], a[x - off:y - off]
+                        if hi - lo <= self.piece:
+                            dst[:] = src
+                        else:
+                            jobs.append(self.pool.submit(np.copyto, dst, src))
+                    if off + a.nbytes <= c1:
+                        k = j + 1
+                    j += 1
+                for f in jobs:
+                    f.result()
+        finally:
+            if nxt is not None:
+                nxt.exception()  # the download ends before the buffer is reused
+
+    def _download(self, host: int, dev: int, nbytes: int) -> None:
+        check(self.lib.utl_memcpy_d2h(self.ctx.handle, host, dev, nbytes), "d2h")
 
     def put(self, dev: int, nbytes: int, srcs: list[tuple[int, np.ndarray]]) -> None:
         """The other direction: array a's bytes to device [dev + off, ...)
@@ -221,7 +245,10 @@ class _HostStage:
                     j += 1
                 for f in jobs:
                     f.result()
-                uploads[h] = self.uploader.submit(self._upload, dev + c0, self.pin.ptr + base, c1 - c0)
+                if c0 == 0 and c1 == nbytes:  # one chunk: nothing to overlap, no thread hop
+                    self._upload(dev, self.pin.ptr + base, c1)
+                else:
+                    uploads[h] = self.mover.submit(self._upload, dev + c0, self.pin.ptr + base, c1 - c0)
                 c0, n = c1, n + 1
         finally:
             for f in uploads:
@@ -236,7 +263,7 @@ class _HostStage:
 
     def close(self) -> None:
         self.pool.shutdown()
-        self.uploader.shutdown()
+        self.mover.shutdown()
         self.pin.free()
 
 
