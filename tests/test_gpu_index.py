@@ -167,3 +167,26 @@ def test_block_lengths_pass_vs_oracle(ctx, orc):
         assert meta[k] == want, (k, src[:8].hex(), len(src))
         seen.add(err)
     assert seen >= {0, 1, 3, 4}  # OK, InvalidBuffer, InvalidBlockLength, InvalidFlags all exercised
+
+
+def test_wave_index_repeated_calls(ctx):
+    """The wave index's single-pass scan keeps its tile ticket and status
+    words in the context's scratch: calls of alternating sizes on one
+    context (the scratch refilled each time, its ticket and flags left from
+    the call before) give every entry exactly."""
+    from utlpfor_amd import _lib
+    from utlpfor_amd.device import DeviceArray, check
+    rng = np.random.default_rng(99)
+    for n in (5 * 1024 * 1024 + 3, 4097, 12 * 1024 * 1024 + 1, 1, 5 * 1024 * 1024 + 3):
+        meta = _meta(rng, n)
+        say(f"wave index again: {n} blocks")
+        dm = DeviceArray.from_numpy(meta, ctx)
+        nw = (n + 7) // 8
+        dw = DeviceArray(2 * nw, np.uint64, ctx)
+        check(_lib.load().utl_wave_index_build(ctx.handle, dm.ptr, n, dw.ptr), "utl_wave_index_build")
+        ctx.sync()
+        wb, wo = _want(meta)
+        got = dw.to_numpy()
+        assert np.array_equal(got[0::2], wb[0:n:8]) and np.array_equal(got[1::2], wo[0:n:8])
+        dm.free()
+        dw.free()
