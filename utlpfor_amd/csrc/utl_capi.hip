@@ -114,21 +114,24 @@ thread_local TlsCtx g_tls;
 // The context to use: the caller's, or this thread's default on device 0.
 utl_ctx* resolve(utl_ctx* c, int* err) {
   *err = UTL_OK;
-  if (c) {
-    if (hipSetDevice(c->device) != hipSuccess) *err = UTL_ERR_HIP;
-    return c;
-  }
-  if (!g_tls.c) {
-    utl_ctx* n = new utl_ctx();
-    *err = ctx_init(n, 0);
-    if (*err) {
-      ctx_free(n);
-      return nullptr;
+  if (!c) {
+    if (!g_tls.c) {
+      utl_ctx* n = new utl_ctx();
+      *err = ctx_init(n, 0);
+      if (*err) {
+        ctx_free(n);
+        return nullptr;
+      }
+      g_tls.c = n;
     }
-    g_tls.c = n;
+    c = g_tls.c;
   }
-  hipSetDevice(g_tls.c->device);
-  return g_tls.c;
+  const hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) {
+    *err = hip_fail(e, "hipSetDevice");
+    return nullptr;
+  }
+  return c;
 }
 
 int ensure_tmp(utl_ctx* c, size_t bytes) {
