@@ -1166,17 +1166,24 @@ __global__ void k_fill_lb(uint64_t* p, uint64_t n) {
 
 // Job table for streams cut into 128-value blocks: stream s covers values
 // [val_off[s], val_off[s+1]) with flag flags[s]; its blocks are numbered from
-// blk_off[s] (blk_off[nstreams] blocks in all, read on the device). A
-// grid-stride loop over blocks; each block's stream by binary search (the
-// last stream starting at or before it, so empty streams are skipped).
+// blk_off[s] (blk_off[nstreams] blocks in all, read on the device). Block
+// k's stream is the last one starting at or before it (so empty streams are
+// skipped). A wavefront takes runs of 1024 consecutive blocks, lane l the
+// blocks base + 64q + l (stores coalesced over the wave): the first by
+// binary search, each next one by a gallop up from the stream of the last.
+constexpr uint64_t kJobRun = 64 * 16;
+
 __global__ __launch_bounds__(256) void k_make_jobs(const uint64_t* __restrict__ val_off,
                                                    const uint64_t* __restrict__ blk_off,
                                                    const uint8_t* __restrict__ flags, uint64_t nstreams,
                                                    uint64_t* __restrict__ in_off, uint32_t* __restrict__ in_meta) {
   const uint64_t nblocks = blk_off[nstreams];
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nblocks; k += step) {
-    uint64_t lo = 0, hi = nstreams - 1;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wave * kJobRun; base < nblocks; base += nwaves * kJobRun) {
+    uint64_t k = base + (uint64_t)lane_id();
+    if (k >= nblocks) continue;
+    uint64_t lo = 0, hi = nstreams - 1;  // the last s with blk_off[s] <= k
     while (lo < hi) {
       const uint64_t mid = (lo + hi + 1) >> 1;
       if (blk_off[mid] <= k)
@@ -1184,11 +1191,31 @@ __global__ __launch_bounds__(256) void k_make_jobs(const uint64_t* __restrict__ 
       else
         hi = mid - 1;
     }
-    const uint64_t s = lo, i = k - blk_off[s];
-    const uint64_t v0 = val_off[s], nv = val_off[s + 1] - v0;
-    const uint64_t rest = nv - 128 * i;
-    in_off[k] = v0 + 128 * i;
-    in_meta[k] = (uint32_t)(rest < 128 ? rest : 128) | ((uint32_t)flags[s] << 8);
+    uint64_t s = lo;
+    for (int q = 0; q < (int)(kJobRun / 64) && k < nblocks; ++q, k += 64) {
+      if (q > 0 && s + 1 < nstreams && blk_off[s + 1] <= k) {
+        // gallop up to a bracket [s, t) with blk_off[s] <= k < blk_off[t]
+        uint64_t t = s + 1, step = 1;
+        while (t < nstreams && blk_off[t] <= k) {
+          s = t;
+          step <<= 1;
+          t = s + step;
+        }
+        uint64_t h = t < nstreams ? t : nstreams;  // blk_off[h] > k or h == nstreams
+        while (h - s > 1) {
+          const uint64_t mid = (s + h) >> 1;
+          if (blk_off[mid] <= k)
+            s = mid;
+          else
+            h = mid;
+        }
+      }
+      const uint64_t i = k - blk_off[s];
+      const uint64_t v0 = val_off[s], nv = val_off[s + 1] - v0;
+      const uint64_t rest = nv - 128 * i;
+      in_off[k] = v0 + 128 * i;
+      in_meta[k] = (uint32_t)(rest < 128 ? rest : 128) | ((uint32_t)flags[s] << 8);
+    }
   }
 }
 
