@@ -5,7 +5,9 @@ encoder's concatenated stream of the 64 Mi delta postings (flag Delta)
 equals the oracle's PackUint32(Delta|Append) stream block by block, and its
 meta words equal the oracle's block lengths and counts; 20,000 uneven
 streams of the mixed-flag generator (326 M values) are compared stream by
-stream (the oracle packs them on a thread pool); the two uint64 workloads
+stream in every encode form (the oracle packs them on a thread pool), and
+all 200,000 (3.27 G values) in the default form, with the graph step's
+decode of every block against the oracle's; the two uint64 workloads
 (32 Mi FOR64 timestamps, 8 Mi uniform two-block values) compare encode bytes
 and the decode of every block (status words and values) with the oracle's.
 (format_conformance_test.go:727-754 is the reference's byte-exact check on
@@ -84,6 +86,52 @@ def test_uneven_20000_streams_encode_bytes(ctx, orc, method):
     assert np.all(st.to_numpy() == 0) and np.array_equal(out.to_numpy(), v)
     out.free()
     st.free()
+    s.free()
+
+
+def test_uneven_200000_streams_whole_workload(ctx, orc):
+    """The headline workload at its full size (200,000 streams, 3.27 G
+    values, 25.7 M blocks, 7.8 GB packed): the GPU encoder's bytes equal the
+    oracle's PackUint32(flag|Append) streams and its meta words their block
+    lengths and counts; the GPU decode of every block (the graph step: wave
+    index + meta-form decode) equals the oracle's UnpackUint32 of every
+    block, values and statuses, and the generated values."""
+    from utlpfor_amd import batch, workloads
+    from utlpfor_amd.device import DeviceArray
+    v, lens, flags = workloads.uneven_streams(200_000, seed=3)
+    vo = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    say(f"uneven: {lens.size} streams, {v.size} values; oracle pack")
+
+    def pack(i):
+        return orc.pack_stream(int(flags[i]), v[vo[i]:vo[i + 1]], nthreads=1)
+
+    with ThreadPoolExecutor(16) as ex:
+        parts = list(ex.map(pack, range(lens.size)))
+    want = np.concatenate([p[0] for p in parts])
+    wmeta = np.concatenate([_meta_of(p[1], p[2]) for p in parts])
+    del parts
+    say("GPU encode")
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    assert s.nbytes == want.size and s.nblocks == wmeta.size
+    assert np.array_equal(s.meta.to_numpy(), wmeta)
+    got = s.bytes_host()
+    assert np.array_equal(got, want)
+    del want, got
+    say("GPU decode (graph step)")
+    out = DeviceArray(v.size, np.uint32, ctx)
+    st = DeviceArray(s.nblocks, np.int32, ctx)
+    g = batch.DecodeGraph(ctx, s, out, st, persistent=False, block_index=False)
+    g.launch()
+    ctx.sync()
+    g.close()
+    gout, gst = out.to_numpy(), st.to_numpy()
+    out.free()
+    st.free()
+    say("oracle decode of every block")
+    wout, wst = orc.unpack_blocks(s.bytes_host(), s.block_off.to_numpy(), s.out_off.to_numpy(), v.size, False,
+                                  nthreads=16)
+    assert np.array_equal(gst, wst) and np.all(wst == 0)
+    assert np.array_equal(gout, wout) and np.array_equal(gout, v)
     s.free()
 
 
