@@ -253,7 +253,7 @@ def test_overflow_positions_gpu(gpu, orc):
             check_get(codec, ctx, orc, blob, row * 16 + lane, False)
 
 
-def test_batched_stream_config1(gpu, orc):
+def test_batched_stream_plain_1m(gpu, orc):
     """1,048,576 values U[0,2^12) with 3% in [2^20, 2^32), flag 0."""
     from utlpfor_amd import batch
     codec, ctx = gpu
@@ -263,12 +263,12 @@ def test_batched_stream_config1(gpu, orc):
     idx = rng.choice(n, round(0.03 * n), replace=False)
     v[idx] = rng.integers(1 << 20, 1 << 32, idx.size, dtype=np.uint64)
     v = v.astype(np.uint32)
-    say("config1: GPU encode")
+    say("plain 1 Mi: GPU encode")
     s = batch.encode(ctx, v, 0)
     want, bl, bc = orc.pack_stream(0, v)
     got = s.bytes_host()
     assert got.size == want.size and np.array_equal(got, want)
-    say("config1: GPU decode")
+    say("plain 1 Mi: GPU decode")
     out, st = batch.decode(ctx, s)
     ctx.sync()
     assert np.all(st.to_numpy() == 0)
@@ -276,7 +276,7 @@ def test_batched_stream_config1(gpu, orc):
     out2, st2 = batch.decode(ctx, s, persistent=True)
     ctx.sync()
     assert np.array_equal(out2.to_numpy(), v) and np.all(st2.to_numpy() == 0)
-    say("config1: Get gather")
+    say("plain 1 Mi: Get gather")
     pos = rng.integers(0, n, 100000).astype(np.int64)
     from utlpfor_amd.device import DeviceArray
     vals, gst = batch.get_batch(ctx, s, DeviceArray.from_numpy(pos, ctx))
