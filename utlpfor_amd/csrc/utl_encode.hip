@@ -48,10 +48,16 @@ __device__ __forceinline__ void lds_fence() {
 constexpr int kEncGrpWaves = 4;
 constexpr int kWaveBlocksEnc = 8;  // blocks per wavefront
 
-// Each block's exceptions by rank (their order by position): the high bits,
-// and the StreamVByte code (bits 0-1) with the position (bits 2-8).
+// Each block's exceptions by rank (their order by position): the high bits
+// (32-bit words from byte kExhOff of the block's sec row), and in exm the
+// offset of the exception's data bytes among the section's data bytes
+// (bits 0-8: at most 127 * 4) and its position (bits 9-15).
+// The sec row also stages the block's exception section before it is stored
+// (tail_write).
+constexpr int kExhOff = 52;   // >= the longest index + control bytes (16 + 32) + 3 alignment bytes + 1
+constexpr int kSecRow = 576;  // kExhOff + 4 * 128 high-bit words, rounded up to 16 bytes
 struct alignas(16) EncGrpLds {
-  uint32_t exh[kWaveBlocksEnc][kBlock];
+  uint8_t sec[kWaveBlocksEnc][kSecRow];
   uint16_t exm[kWaveBlocksEnc][kBlock];
 };
 static_assert(sizeof(EncGrpLds) * kEncGrpWaves <= 32768, "LDS per workgroup");
@@ -189,9 +195,10 @@ __device__ __forceinline__ void enc_width(const uint32_t* x0, const uint32_t* x1
 // high bits of each exception, little endian, in position order). A rank is
 // the number of exceptions at lower positions. tail_plan ranks the
 // exceptions (one pass over the positions, writing each one's high bits and
-// code to LDS by rank) and sizes the section; tail_write writes it at any
-// byte address in rounds of eight ranks, rank 8q + j to thread j, the data
-// offsets by a scan over the group.
+// data offset and position to LDS by rank, the offsets from scans of
+// per-row byte counts over the group) and sizes the section; tail_write assembles
+// the section in LDS in rounds of eight ranks, rank 8q + j to thread j, then
+// stores it at its byte address as aligned dwords.
 struct Tail {
   uint32_t bmw;  // bitmap word j >> 1 (bitmap bytes 2j, 2j+1)
   int isz, ctrl, svb_len;
@@ -203,76 +210,141 @@ __device__ __forceinline__ void tail_plan(EncGrpLds* L, int blk, int j, const ui
                                           const bool* ok0, const bool* ok1, int bw, int exc, Tail& t) {
   const uint32_t m = low_mask(bw);
   // bitmap of exception positions: word d holds positions 32d .. 32d + 31
-  // (rows 2d, 2d+1; bit 16 (r & 1) + 2j + e)
+  // (rows 2d, 2d+1; bit 16 (r & 1) + 2j + e), this thread's bits gathered
+  // at 16 (r & 1) + e and moved to 2j once
   uint32_t bm[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
-    const int bit = 16 * (r & 1) + 2 * j;
-    bm[r >> 1] |= ((ok0[r] && x0[r] > m) ? 1u : 0u) << bit;
-    bm[r >> 1] |= ((ok1[r] && x1[r] > m) ? 1u : 0u) << (bit + 1);
+    bm[r >> 1] |= (ok0[r] && x0[r] > m) ? 1u << (16 * (r & 1)) : 0u;
+    bm[r >> 1] |= (ok1[r] && x1[r] > m) ? 2u << (16 * (r & 1)) : 0u;
   }
 #pragma unroll
-  for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d]);
+  for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d] << (2 * j));
   const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
-  uint32_t* exh = L->exh[blk];
-  uint16_t* exm = L->exm[blk];
-  uint32_t codes = 0;  // this thread's exceptions' codes, summed
+  // bytes of the high bits x >> bw of an exception x >= 2^bw: (bit_len(x) - bw + 7) >> 3
+  const int nb_k = 39 - bw;
+  // Data bytes (code + 1 each) of this thread's exceptions in row r: byte r
+  // of lA (rows 0-3) or byte r - 4 of lB, those of its first value in fA,
+  // fB. A row holds at most 16 exceptions, so every per-row sum below stays
+  // <= 64 and fits its byte.
+  uint32_t lA = 0, lB = 0, fA = 0, fB = 0;
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
-    const int d = r >> 1, bit = 16 * (r & 1) + 2 * j;
+    const bool e0 = ok0[r] && x0[r] > m, e1 = ok1[r] && x1[r] > m;
+    const uint32_t l0 = e0 ? (uint32_t)(nb_k - __clz((int)x0[r])) >> 3 : 0u;
+    const uint32_t l = l0 + (e1 ? (uint32_t)(nb_k - __clz((int)x1[r])) >> 3 : 0u);
+    if (r < 4) {
+      lA |= l << (8 * r);
+      fA |= l0 << (8 * r);
+    } else {
+      lB |= l << (8 * (r - 4));
+      fB |= l0 << (8 * (r - 4));
+    }
+  }
+  // per row: the data bytes of the group's threads before this one, and the
+  // row's total; then the rows before each row (bytes 0-3 of PA and PB, at
+  // most 3 * 64, so no byte carries), sA and sB the totals of rows 0-3, 4-7
+  const uint32_t xA = g8_incl_scan(lA) - lA, xB = g8_incl_scan(lB) - lB;
+  const uint32_t tA = g8_add(lA), tB = g8_add(lB);
+  const uint32_t PA = tA * 0x01010100u, PB = tB * 0x01010100u;
+  const uint32_t sA = (PA >> 24) + (tA >> 24), sB = (PB >> 24) + (tB >> 24);
+  const uint32_t oA = PA + xA, oB = PB + xB;  // per row: data bytes before this thread's first value (fields <= 255)
+  uint32_t* exh = reinterpret_cast<uint32_t*>(L->sec[blk] + kExhOff);
+  uint16_t* exm = L->exm[blk];
+  const uint32_t below0 = (1u << (2 * j)) - 1u, below1 = (0x10000u << (2 * j)) - 1u;  // bits before 2j, 16 + 2j
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int d = r >> 1;
     const uint32_t pd = d == 0 ? 0u : d == 1 ? pre1 : d == 2 ? pre2 : pre3;
-    const uint32_t rank0 = pd + __popc(bm[d] & ((1u << bit) - 1u));
-    const bool e0 = (bm[d] >> bit) & 1u, e1 = (bm[d] >> (bit + 1)) & 1u;
+    const uint32_t rank0 = pd + __popc(bm[d] & ((r & 1) ? below1 : below0));
+    const bool e0 = ok0[r] && x0[r] > m, e1 = ok1[r] && x1[r] > m;
+    const uint32_t D0 = r < 4 ? (oA >> (8 * r)) & 0xFFu : sA + ((oB >> (8 * (r - 4))) & 0xFFu);
+    uint32_t v0 = x0[r], v1 = x1[r];
+    asm volatile("" : "+v"(v0), "+v"(v1));  // shift again here rather than keep the first pass's results live
     if (e0) {
-      const uint32_t h = x0[r] >> bw, c = svb_code(h);
-      exh[rank0] = h;
-      exm[rank0] = (uint16_t)(c | ((uint32_t)(16 * r + 2 * j) << 2));
-      codes += c;
+      exh[rank0] = v0 >> bw;
+      exm[rank0] = (uint16_t)(D0 | ((uint32_t)(16 * r + 2 * j) << 9));
     }
     if (e1) {
-      const uint32_t h = x1[r] >> bw, c = svb_code(h), rank1 = rank0 + (e0 ? 1u : 0u);
-      exh[rank1] = h;
-      exm[rank1] = (uint16_t)(c | ((uint32_t)(16 * r + 2 * j + 1) << 2));
-      codes += c;
+      const uint32_t l0 = r < 4 ? (fA >> (8 * r)) & 0xFFu : (fB >> (8 * (r - 4))) & 0xFFu;
+      const uint32_t rank1 = rank0 + (e0 ? 1u : 0u);
+      exh[rank1] = v1 >> bw;
+      exm[rank1] = (uint16_t)((D0 + l0) | ((uint32_t)(16 * r + 2 * j + 1) << 9));
     }
   }
   t.isz = exc_index_size(exc);
   t.ctrl = svb_ctrl_count(exc);
-  t.svb_len = t.ctrl + exc + (int)g8_add(codes);  // a data byte per exception and one per code step
+  t.svb_len = t.ctrl + (int)(sA + sB);
   t.bmw = (j >> 1) == 0 ? bm[0] : (j >> 1) == 1 ? bm[1] : (j >> 1) == 2 ? bm[2] : bm[3];
   lds_fence();
 }
 
-__device__ __forceinline__ void tail_write(const EncGrpLds* L, int blk, int j, int exc, const Tail& t,
+// The section is staged in the block's sec row from byte L0 on, where L0 is
+// congruent to T modulo 4, so that the stored dwords are LDS dwords. L0 plus
+// the index and control bytes is at most kExhOff: round q writes data bytes
+// below byte kExhOff + 4 (8q + 8) (each of the first 8q + 8 ranks has at
+// most 4), where the high-bit words of the later rounds begin, and it reads
+// its own ranks' words before it writes.
+__device__ __forceinline__ void tail_write(EncGrpLds* L, int blk, int j, int exc, const Tail& t,
                                            uint8_t* __restrict__ g, uint64_t T) {
-  const uint32_t* exh = L->exh[blk];
+  uint8_t* sec = L->sec[blk];
+  const uint32_t* exh = reinterpret_cast<const uint32_t*>(sec + kExhOff);
   const uint16_t* exm = L->exm[blk];
-  if (exc > kExcBitmap) {  // bitmap bytes 2j, 2j+1 (position 8q + i is bit i of byte q)
-    g[T + 2 * j] = (uint8_t)(t.bmw >> (16 * (j & 1)));
-    g[T + 2 * j + 1] = (uint8_t)(t.bmw >> (16 * (j & 1) + 8));
+  const int hdr = t.isz + t.ctrl;  // index and control bytes
+  const int a = (int)(T & 3);
+  const int L0 = (kExhOff - hdr) - (((kExhOff - hdr) - a) & 3);
+  const bool list = exc <= kExcBitmap;
+  if (!list) {  // bitmap bytes 2j, 2j+1 (position 8q + i is bit i of byte q)
+    sec[L0 + 2 * j] = (uint8_t)(t.bmw >> (16 * (j & 1)));
+    sec[L0 + 2 * j + 1] = (uint8_t)(t.bmw >> (16 * (j & 1) + 8));
   }
-  const uint64_t cbase = T + (uint64_t)t.isz, dbase = cbase + (uint64_t)t.ctrl;
+  const int cbase = L0 + t.isz, dbase = L0 + hdr;
   const int rounds = (exc + 7) >> 3;
-  uint32_t run = 0;  // data bytes of the rounds before
+  uint32_t e = exm[j], h = exh[j];  // rank j (rank 8q + j in round q; every rank read is < 128)
 #pragma unroll 1
   for (int q = 0; q < rounds; ++q) {
     const int k = 8 * q + j;
+    const int kn = k + 8 < kBlock ? k + 8 : k;
+    const uint32_t en = exm[kn], hn = exh[kn];
     const bool valid = k < exc;
-    const uint32_t mk = valid ? (uint32_t)exm[k] : 0u, h = valid ? exh[k] : 0u;
-    const uint32_t c = mk & 3u, l = valid ? c + 1u : 0u;
-    const uint32_t incl = g8_incl_scan(l);
-    const uint64_t at = dbase + run + (incl - l);
-    run += (uint32_t)__shfl((int)incl, lane_id() | 7, 64);
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if ((uint32_t)b < l) g[at + b] = (uint8_t)(h >> (8 * b));
-    if (exc <= kExcBitmap && valid) g[T + k] = (uint8_t)(mk >> 2);  // the position list
+    const uint32_t c = valid ? svb_code(h) : 0u;
+    const int at = dbase + (int)(e & 0x1FFu);
+    if (valid) {
+      sec[at] = (uint8_t)h;
+      if (c >= 1) sec[at + 1] = (uint8_t)(h >> 8);
+      if (c >= 2) sec[at + 2] = (uint8_t)(h >> 16);
+      if (c >= 3) sec[at + 3] = (uint8_t)(h >> 24);
+      if (list) sec[L0 + k] = (uint8_t)(e >> 9);  // the position list
+    }
     // control bytes 2q, 2q + 1: the codes of threads 0..3 and 4..7
     uint32_t cb = c << (2 * (j & 3));
     cb |= g8_dpp<0xB1>(cb);
     cb |= g8_dpp<0x4E>(cb);
     const int ci = 2 * q + (j >> 2);
-    if ((j & 3) == 0 && ci < t.ctrl) g[cbase + ci] = (uint8_t)cb;
+    if ((j & 3) == 0 && ci < t.ctrl) sec[cbase + ci] = (uint8_t)cb;
+    e = en;
+    h = hn;
+  }
+  lds_fence();
+  // the section's dwords: whole ones as dword stores, the partial ones at
+  // its two ends byte by byte by threads 0 and 1 (their other bytes belong
+  // to the payload before it and to the next block)
+  const int e_end = a + t.isz + t.svb_len;  // one past the section, from dword 0
+  const int i0 = a ? 1 : 0, i1 = e_end >> 2;
+  const uint32_t* s32 = reinterpret_cast<const uint32_t*>(sec + (L0 - a));
+  uint8_t* g0 = g + (T - (uint64_t)a);
+  uint32_t* g32 = reinterpret_cast<uint32_t*>(g0);
+#pragma unroll 1
+  for (int i = i0 + j; i < i1; i += 8) g32[i] = s32[i];
+  // head: bytes [a, min(4, e_end)) of dword 0; tail: bytes [0, e_end & 3) of
+  // dword i1 when that is not dword 0 as well
+  const bool head = j == 0 && a != 0, tail = j == 1 && (e_end & 3) != 0 && (a == 0 || i1 > 0);
+  if (head || tail) {
+    const int di = head ? 0 : i1, lo = head ? a : 0, hi = head ? min(4, e_end) : (e_end & 3);
+    const uint32_t w = s32[di];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (lo + b < hi) g0[4 * di + lo + b] = (uint8_t)(w >> (8 * (lo + b)));
   }
 }
 
