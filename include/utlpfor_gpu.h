@@ -263,16 +263,22 @@ int utl_get_batch(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t
 
 /* Position directory for batched Gets by stream position (no reference
  * counterpart; the reference's callers find a block by walking BlockLength):
- * d_dir[c] = the last block whose d_out_off entry is <= 128c (0 when none),
- * for c < ndir; ndir = ceil(d_out_off[nblocks] / 128) entries cover every
- * position of the stream. d_out_off must be nondecreasing. */
-int utl_pos_dir_build(utl_ctx *ctx, const uint64_t *d_out_off, size_t nblocks, uint64_t *d_dir, size_t ndir);
+ * entry c (UTL_POS_DIR_WORDS uint64 words of d_dir) describes the last block
+ * whose d_out_off entry is <= 128c (0 when none) and the block after it:
+ * their d_block_off / d_out_off entries, for c < ndir; ndir =
+ * ceil(d_out_off[nblocks] / 128) entries cover every position of the
+ * stream. d_out_off must be nondecreasing; the directory holds both arrays'
+ * values and is rebuilt when either changes. */
+#define UTL_POS_DIR_WORDS 4
+int utl_pos_dir_build(utl_ctx *ctx, const uint64_t *d_block_off, const uint64_t *d_out_off, size_t nblocks,
+                      uint64_t *d_dir, size_t ndir);
 
 /* utl_get_batch of stream positions (get.go:74, uint64.go:364) with the
- * block search started at d_dir[position / 128] from utl_pos_dir_build over
- * the same d_out_off: the same values and statuses as utl_get_batch with
- * d_req_block NULL, with one directory read and usually one d_out_off line
- * per request instead of a gallop over d_out_off. */
+ * block found through d_dir (ndir entries) from utl_pos_dir_build over the
+ * same d_block_off and d_out_off: the same values and statuses as
+ * utl_get_batch with d_req_block NULL. A position in an entry's block or the
+ * next takes their offsets from the entry (one directory read per request);
+ * others gallop over d_out_off from the entry's block. */
 int utl_get_batch_dir(utl_ctx *ctx, int is_u64, const uint8_t *d_src, const uint64_t *d_block_off,
                       const uint64_t *d_out_off, size_t nblocks, const uint64_t *d_dir, size_t ndir,
                       const int64_t *d_req_pos, size_t nreq, void *d_out, int32_t *d_status);

@@ -92,9 +92,10 @@ def test_get_dir_rejects_bad_arguments_before_device_work(lib):
     dereferenced."""
     fake = 0x100000
     err = 100  # UTL_ERR_ARGUMENT
-    assert lib.utl_pos_dir_build(None, None, 4, fake, 1) == err
-    assert lib.utl_pos_dir_build(None, fake, 4, None, 1) == err
-    assert lib.utl_pos_dir_build(None, fake, 0, fake, 1) == err
+    assert lib.utl_pos_dir_build(None, None, fake, 4, fake, 1) == err
+    assert lib.utl_pos_dir_build(None, fake, None, 4, fake, 1) == err
+    assert lib.utl_pos_dir_build(None, fake, fake, 4, None, 1) == err
+    assert lib.utl_pos_dir_build(None, fake, fake, 0, fake, 1) == err
     assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, None, 1, fake, 1, fake, fake) == err
     assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, fake, 0, fake, 1, fake, fake) == err
     assert lib.utl_get_batch_dir(None, 0, fake, fake, fake, 4, fake, 1, None, 1, fake, fake) == err

@@ -57,17 +57,35 @@ def _get(ctx, lib, s, pos, d=None, ndir=None):
     return r
 
 
-def _dir(ctx, lib, ooff_dev, nblocks, ndir):
+def _dir(ctx, lib, boff_dev, ooff_dev, nblocks, ndir):
     from utlpfor_amd.device import DeviceArray, check
-    d = DeviceArray(ndir, np.uint64, ctx)
-    check(lib.utl_pos_dir_build(ctx.handle, ooff_dev.ptr, nblocks, d.ptr, ndir), "pos_dir")
+    d = DeviceArray(ndir * 4, np.uint64, ctx)
+    check(lib.utl_pos_dir_build(ctx.handle, boff_dev.ptr, ooff_dev.ptr, nblocks, d.ptr, ndir), "pos_dir")
     ctx.sync()
     return d
 
 
-def _want_dir(ooff, ndir):
+def _want_dir(boff, ooff, ndir):
+    """Directory entries as 4 words each: block_off and out_off of the last
+    block b whose out_off entry is <= 128c (0 when none), b, then the byte
+    lengths (16 bits each) and value counts (8 bits each) of b and b + 1 with
+    their usable bits (16, 17 of the upper half), the differences taken as
+    unsigned 64-bit numbers."""
+    nb = ooff.size - 1
     c = np.arange(ndir, dtype=np.uint64) * 128
-    return np.maximum(np.searchsorted(ooff[:-1], c, side="right") - 1, 0).astype(np.uint64)
+    b = np.maximum(np.searchsorted(ooff[:-1], c, side="right") - 1, 0).astype(np.int64)
+    l0 = boff[b + 1] - boff[b]
+    n0 = ooff[b + 1] - ooff[b]
+    use0 = (l0 <= 0xFFFF) & (n0 <= 0xFF)
+    has1 = b + 1 < nb
+    b2 = np.minimum(b + 2, nb)
+    l1 = np.where(has1, boff[b2] - boff[np.minimum(b + 1, nb)], 0).astype(np.uint64)
+    n1 = np.where(has1, ooff[b2] - ooff[np.minimum(b + 1, nb)], 0).astype(np.uint64)
+    use1 = use0 & has1 & (l1 <= 0xFFFF) & (n1 <= 0xFF)
+    z = np.uint64(0)
+    lens = np.where(use0, l0, z) | np.where(use1, l1 << np.uint64(16), z)
+    cnts = (np.where(use0, n0 | np.uint64(1 << 16), z) | np.where(use1, (n1 << np.uint64(8)) | np.uint64(1 << 17), z))
+    return np.stack([boff[b], ooff[b], b.astype(np.uint64), lens | (cnts << np.uint64(32))], axis=1).reshape(-1)
 
 
 @pytest.mark.parametrize("is64", [False, True])
@@ -82,7 +100,7 @@ def test_dir_get_equals_search_and_values(ctx, is64):
     assert total == v.size
     ndir = (total + 127) // 128
     d = s.ensure_pos_dir()
-    assert d.n == ndir and np.array_equal(d.to_numpy(), _want_dir(ooff, ndir))
+    assert d.n == 4 * ndir and np.array_equal(d.to_numpy(), _want_dir(s.block_off.to_numpy(), ooff, ndir))
     rng = np.random.default_rng(5)
     edges = np.concatenate([ooff[:-1], ooff[1:] - 1, [0, total - 1, -1, -7, total, total + 1, total + 300]])
     pos = np.concatenate([rng.integers(0, total, 200000), edges.astype(np.int64)]).astype(np.int64)
@@ -132,12 +150,50 @@ def test_dir_over_arbitrary_out_off(ctx):
         g_srch, s_srch = _get(ctx, lib, s2, pos)
         for ndir in ((total + 127) // 128, 1, max((total + 127) // 128 // 3, 1), (total + 127) // 128 + 50):
             say(f"trial {trial}: first {first}, total {total}, ndir {ndir}")
-            d = _dir(ctx, lib, s2.out_off, s2.nblocks, ndir)
-            assert np.array_equal(d.to_numpy(), _want_dir(ooff, ndir))
+            d = _dir(ctx, lib, s2.block_off, s2.out_off, s2.nblocks, ndir)
+            assert np.array_equal(d.to_numpy(), _want_dir(s2.block_off.to_numpy(), ooff, ndir))
             g_dir, s_dir = _get(ctx, lib, s2, pos, d, ndir)
             assert np.array_equal(s_dir, s_srch) and np.array_equal(g_dir, g_srch)
             d.free()
         s2.out_off.free()
+    s.free()
+
+
+def test_dir_over_blocks_with_long_byte_ranges(ctx):
+    """block_off ranges longer than a directory entry's 16-bit length field:
+    padding after some blocks (a caller's layout with gaps), so those blocks
+    and the ones before them take the search; every Get equals the
+    directory-less search's and the values are the stream's."""
+    from utlpfor_amd import _lib, batch
+    from utlpfor_amd.device import DeviceArray
+    lib = _lib.load()
+    v, lens, flags = _streams(43, 300, 4000)
+    s = batch.encode(ctx, v, stream_lengths=lens, stream_flags=flags)
+    data = s.data.to_numpy(s.nbytes)
+    boff = s.block_off.to_numpy()
+    rng = np.random.default_rng(12)
+    pad = np.where(rng.random(s.nblocks) < 0.05, rng.integers(60000, 70000, s.nblocks), 0).astype(np.uint64)
+    parts = []
+    for b in range(s.nblocks):
+        parts.append(data[boff[b]:boff[b + 1]])
+        parts.append(np.zeros(int(pad[b]), np.uint8))
+    nboff = np.concatenate([[0], np.cumsum(np.diff(boff) + pad)]).astype(np.uint64)
+    ndata = np.concatenate(parts)
+    say(f"long block ranges: {s.nblocks} blocks, {int((pad > 0).sum())} padded, {ndata.size} bytes")
+    d_data, d_boff = DeviceArray.from_numpy(ndata, ctx), DeviceArray.from_numpy(nboff, ctx)
+    s2 = batch.DeviceStream(ctx, d_data, s.meta, d_boff, s.out_off, s.nblocks, s.nvalues, ndata.size, False)
+    total = s.nvalues
+    ndir = (total + 127) // 128
+    d = _dir(ctx, lib, s2.block_off, s2.out_off, s2.nblocks, ndir)
+    assert np.array_equal(d.to_numpy(), _want_dir(nboff, s.out_off.to_numpy(), ndir))
+    pos = np.concatenate([rng.integers(0, total, 100000), [0, total - 1]]).astype(np.int64)
+    g_dir, s_dir = _get(ctx, lib, s2, pos, d, ndir)
+    g_srch, s_srch = _get(ctx, lib, s2, pos)
+    assert np.array_equal(s_dir, s_srch) and np.array_equal(g_dir, g_srch)
+    good = s_dir == 0
+    assert good.sum() >= 0.5 * pos.size and np.array_equal(g_dir[good], v[pos[good]])
+    for a in (d, d_data, d_boff):
+        a.free()
     s.free()
 
 

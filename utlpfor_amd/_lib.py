@@ -56,7 +56,7 @@ SIGNATURES = {
     "utl_set_tune": (C.c_int, [vp, vp]),
     "utl_get_tune": (C.c_int, [vp, vp]),
     "utl_get_batch": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, vp, sz, vp, vp]),
-    "utl_pos_dir_build": (C.c_int, [vp, vp, sz, vp, sz]),
+    "utl_pos_dir_build": (C.c_int, [vp, vp, vp, sz, vp, sz]),
     "utl_get_batch_dir": (C.c_int, [vp, C.c_int, vp, vp, vp, sz, vp, sz, vp, sz, vp, vp]),
     "utl_gather_index": (C.c_int, [vp, vp, vp, sz, vp, sz, sz, vp, vp]),
     "utl_index_walk": (C.c_int, [vp, vp, vp, sz, vp, vp, vp, vp, vp]),

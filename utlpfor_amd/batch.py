@@ -20,6 +20,7 @@ from .device import Context, DeviceArray, check
 SLOT32, SLOT64 = 1024, 2048
 MAX_BLOCK32 = 1018  # MaxBlockLength32(0), the longest uint32 block (scalar.go:336-357)
 MAX_BLOCK64 = 2038  # MaxBlockLength64(0), the longest uint64 block (uint64.go:451-453)
+POS_DIR_WORDS = 4  # uint64 words per position-directory entry (UTL_POS_DIR_WORDS)
 
 
 @dataclass
@@ -39,14 +40,15 @@ class DeviceStream:
         return self.data.to_numpy(self.nbytes)
 
     def ensure_pos_dir(self) -> DeviceArray:
-        """The position directory over out_off (utl_pos_dir_build): entry c is
-        the block holding value position 128c."""
+        """The position directory over block_off / out_off (utl_pos_dir_build):
+        entry c (POS_DIR_WORDS words) describes the block holding value
+        position 128c and the block after it."""
         if self.pos_dir is None:
             lib = _lib.load()
             ndir = max((self.nvalues + 127) // 128, 1)
-            d = DeviceArray(ndir, np.uint64, self.ctx)
-            check(lib.utl_pos_dir_build(self.ctx.handle, self.out_off.ptr, self.nblocks, d.ptr, ndir),
-                  "utl_pos_dir_build")
+            d = DeviceArray(ndir * POS_DIR_WORDS, np.uint64, self.ctx)
+            check(lib.utl_pos_dir_build(self.ctx.handle, self.block_off.ptr, self.out_off.ptr, self.nblocks, d.ptr,
+                                        ndir), "utl_pos_dir_build")
             self.pos_dir = d
         return self.pos_dir
 
@@ -282,7 +284,8 @@ def get_batch(ctx: Context, s: DeviceStream, positions: DeviceArray, out: Device
     if blocks is None and use_dir and s.nblocks > 0:
         d = s.ensure_pos_dir()
         check(lib.utl_get_batch_dir(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
-                                    d.ptr, d.n, positions.ptr, n, out.ptr, status.ptr), "utl_get_batch_dir")
+                                    d.ptr, d.n // POS_DIR_WORDS, positions.ptr, n, out.ptr, status.ptr),
+              "utl_get_batch_dir")
         return out, status
     check(lib.utl_get_batch(ctx.handle, int(s.is64), s.data.ptr, s.block_off.ptr, s.out_off.ptr, s.nblocks,
                             blocks.ptr if blocks is not None else None, positions.ptr, n, out.ptr, status.ptr),

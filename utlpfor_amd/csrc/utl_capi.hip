@@ -629,12 +629,13 @@ int utl_get_batch(utl_ctx* ctx, int is_u64, const uint8_t* d_src, const uint64_t
   return UTL_OK;
 }
 
-int utl_pos_dir_build(utl_ctx* ctx, const uint64_t* d_out_off, size_t nblocks, uint64_t* d_dir, size_t ndir) {
-  if (ndir && (!d_out_off || !d_dir || nblocks == 0)) return UTL_ERR_ARGUMENT;
+int utl_pos_dir_build(utl_ctx* ctx, const uint64_t* d_block_off, const uint64_t* d_out_off, size_t nblocks,
+                      uint64_t* d_dir, size_t ndir) {
+  if (ndir && (!d_block_off || !d_out_off || !d_dir || nblocks == 0)) return UTL_ERR_ARGUMENT;
   int err;
   utl_ctx* c = resolve(ctx, &err);
   if (!c) return err;
-  HIP_TRY(utl_launch_pos_dir(d_out_off, nblocks, d_dir, ndir, c->stream));
+  HIP_TRY(utl_launch_pos_dir(d_block_off, d_out_off, nblocks, d_dir, ndir, c->stream));
   return UTL_OK;
 }
 

@@ -485,11 +485,26 @@ __device__ __forceinline__ void job_resume(GetJob& j, int st, uint32_t v) {
   }
 }
 
+// A position-directory entry (k_pos_dir): the block b holding position
+// 128c and what a Get needs of blocks b and b + 1 without reading the offset
+// arrays. lens: boff[b + 1] - boff[b] | (boff[b + 2] - boff[b + 1]) << 16;
+// cnts: ooff[b + 1] - ooff[b] | (ooff[b + 2] - ooff[b + 1]) << 8 |
+// kDirUse0 when block b's fields fit | kDirUse1 when block b + 1's do too.
+struct alignas(16) PosDirEntry {
+  uint64_t boff0, ooff0, b;
+  uint32_t lens, cnts;
+};
+static_assert(sizeof(PosDirEntry) == 4 * sizeof(uint64_t), "four words per directory entry");
+constexpr uint32_t kDirUse0 = 1u << 16, kDirUse1 = 1u << 17;
+
 // Requests: either (req_block[i], req_pos[i]) pairs, or stream value
 // positions req_pos[i], each mapped to the last block b with ooff[b] <= pos.
-// With a position directory (dir != nullptr, see k_pos_dir) the search
-// starts at dir[pos / 128], the block holding position pos & ~127, so it
-// usually reads one out_off line; without one it gallops from pos / 128.
+// With a position directory (dir != nullptr, ndir entries of PosDirEntry)
+// a position in the entry's block or the next one takes that block's
+// offsets from the entry (with a nondecreasing ooff, ooff[b] <= pos <
+// ooff[b + 1] makes b the last block whose entry is <= pos); otherwise the
+// search gallops from the entry's block. Without one it gallops from
+// pos / 128.
 // Two launches over the same requests. The first (FULL false) runs each
 // request's Get on one thread and marks the requests that reach the
 // reference's full-unpack fallback with status kGetDeferStatus. The second
@@ -504,7 +519,7 @@ constexpr int32_t kGetDeferStatus = -1;
 template <bool U64, bool FULL>
 __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, const uint64_t* __restrict__ boff,
                                              const uint64_t* __restrict__ ooff, uint64_t nblocks,
-                                             const uint64_t* __restrict__ dir, uint64_t ndir,
+                                             const PosDirEntry* __restrict__ dir, uint64_t ndir,
                                              const uint64_t* __restrict__ req_block, const int64_t* __restrict__ req_pos,
                                              uint64_t nreq, void* __restrict__ out, int32_t* __restrict__ status,
                                              bool all) {
@@ -522,6 +537,9 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
     int64_t pos = req_pos[i];
     uint64_t b;
     int st;
+    // the block's bytes and length when the directory entry gave them
+    const uint8_t* blk = nullptr;
+    int64_t blen = 0;
     if (req_block) {
       b = req_block[i];
       st = b < nblocks ? UTL_OK : UTL_ERR_ARGUMENT;
@@ -533,16 +551,33 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
       // the same b as a bisection of the whole array.
       uint64_t lo = 0, hi = nblocks;
       if (dir) {
-        // dir[c] is the largest b with ooff[b] <= 128c (or 0), and 128c <=
-        // pos: gallop up from it
+        // entry c's block is the largest b with ooff[b] <= 128c (or 0), and
+        // 128c <= pos: the entry's block or the next, else gallop up from it
         const uint64_t c = st == UTL_OK ? (uint64_t)pos >> 7 : 0;
-        lo = c < ndir ? min(dir[c], nblocks - 1) : 0;
-        for (uint64_t step = 1; lo + step < nblocks; step <<= 1) {
-          if (ooff[lo + step] > (uint64_t)pos) {
-            hi = lo + step;
-            break;
+        if (c < ndir) {
+          const PosDirEntry e = dir[c];
+          const uint64_t up = (uint64_t)pos, c0 = e.cnts & 0xFFu, c1 = (e.cnts >> 8) & 0xFFu;
+          if (st == UTL_OK && up >= e.ooff0 && (e.cnts & kDirUse0) && up < e.ooff0 + c0) {
+            b = e.b;
+            blk = src + e.boff0;
+            blen = (int64_t)(e.lens & 0xFFFFu);
+            pos = (int64_t)(up - e.ooff0);
+          } else if (st == UTL_OK && up >= e.ooff0 && (e.cnts & kDirUse1) && up < e.ooff0 + c0 + c1) {
+            b = e.b + 1;
+            blk = src + e.boff0 + (e.lens & 0xFFFFu);
+            blen = (int64_t)(e.lens >> 16);
+            pos = (int64_t)(up - e.ooff0 - c0);
           }
-          lo += step;
+          lo = min(e.b, nblocks - 1);
+        }
+        if (!blk) {
+          for (uint64_t step = 1; lo + step < nblocks; step <<= 1) {
+            if (ooff[lo + step] > (uint64_t)pos) {
+              hi = lo + step;
+              break;
+            }
+            lo += step;
+          }
         }
       } else {
         // Blocks of an index built from meta words hold at most 128 values,
@@ -569,16 +604,25 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
           }
         }
       }
-      while (hi - lo > 1) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (ooff[mid] <= (uint64_t)pos) lo = mid;
-        else hi = mid;
+      if (!blk) {
+        while (hi - lo > 1) {
+          const uint64_t mid = (lo + hi) >> 1;
+          if (ooff[mid] <= (uint64_t)pos) lo = mid;
+          else hi = mid;
+        }
+        b = lo;
+        pos -= (int64_t)ooff[b];
       }
-      b = lo;
-      pos -= (int64_t)ooff[b];
     }
-    if (st == UTL_OK) job_start<U64>(job, src + boff[b], (int64_t)(boff[b + 1] - boff[b]), pos);
-    else job_done(job, st, 0);
+    if (st == UTL_OK) {
+      if (!blk) {
+        blk = src + boff[b];
+        blen = (int64_t)(boff[b + 1] - boff[b]);
+      }
+      job_start<U64>(job, blk, blen, pos);
+    } else {
+      job_done(job, st, 0);
+    }
   }
   if (!FULL) {
     if (live) {
@@ -625,19 +669,37 @@ __global__ __launch_bounds__(256) void k_get(const uint8_t* __restrict__ src, co
   }
 }
 
-// Position directory: dir[c] = the largest block b with ooff[b] <= 128c
-// (0 when there is none), for c < ndir. Thread b writes the entries c with
-// 128c in [ooff[b], ooff[b + 1]), widened to [0, ooff[1]) for block 0 and
-// to [ooff[b], 128 ndir) for the last block; with a nondecreasing ooff these
-// ranges tile [0, 128 ndir), so every entry is written exactly once, and
-// the block of each range is the largest whose offset is <= any position in
-// it (block 0 also where no offset is).
-__global__ __launch_bounds__(256) void k_pos_dir(const uint64_t* __restrict__ ooff, uint64_t nblocks,
-                                                 uint64_t* __restrict__ dir, uint64_t ndir) {
+// Position directory: entry c describes the largest block b with ooff[b]
+// <= 128c (0 when there is none), for c < ndir. Thread b writes the entries
+// c with 128c in [ooff[b], ooff[b + 1]), widened to [0, ooff[1]) for block 0
+// and to [ooff[b], 128 ndir) for the last block; with a nondecreasing ooff
+// these ranges tile [0, 128 ndir), so every entry is written exactly once,
+// and the block of each range is the largest whose offset is <= any
+// position in it (block 0 also where no offset is). A block's fields are
+// marked usable when its byte length and value count fit them (differences
+// of the offset arrays, as unsigned numbers).
+__global__ __launch_bounds__(256) void k_pos_dir(const uint64_t* __restrict__ boff, const uint64_t* __restrict__ ooff,
+                                                 uint64_t nblocks, PosDirEntry* __restrict__ dir, uint64_t ndir) {
   const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
   const uint64_t lo = b == 0 ? 0 : ooff[b], hi = b + 1 == nblocks ? (ndir << 7) : ooff[b + 1];
-  for (uint64_t c = (lo + 127) >> 7; c < ndir && (c << 7) < hi; ++c) dir[c] = b;
+  uint64_t c = (lo + 127) >> 7;
+  if (c >= ndir || (c << 7) >= hi) return;
+  PosDirEntry e;
+  e.boff0 = boff[b];
+  e.ooff0 = ooff[b];
+  e.b = b;
+  const uint64_t l0 = boff[b + 1] - e.boff0, n0 = ooff[b + 1] - e.ooff0;
+  const bool use0 = l0 <= 0xFFFFu && n0 <= 0xFFu;
+  uint64_t l1 = 0, n1 = 0;
+  if (b + 1 < nblocks) {
+    l1 = boff[b + 2] - boff[b + 1];
+    n1 = ooff[b + 2] - ooff[b + 1];
+  }
+  const bool use1 = use0 && b + 1 < nblocks && l1 <= 0xFFFFu && n1 <= 0xFFu;
+  e.lens = (use0 ? (uint32_t)l0 : 0u) | (use1 ? (uint32_t)l1 << 16 : 0u);
+  e.cnts = (use0 ? (uint32_t)n0 | kDirUse0 : 0u) | (use1 ? ((uint32_t)n1 << 8) | kDirUse1 : 0u);
+  for (; c < ndir && (c << 7) < hi; ++c) dir[c] = e;
 }
 
 }  // namespace utl
@@ -645,9 +707,10 @@ __global__ __launch_bounds__(256) void k_pos_dir(const uint64_t* __restrict__ oo
 using namespace utl;
 
 hipError_t utl_launch_get(bool u64, const uint8_t* src, const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks,
-                          const uint64_t* dir, uint64_t ndir, const uint64_t* req_block, const int64_t* req_pos,
+                          const uint64_t* dir_words, uint64_t ndir, const uint64_t* req_block, const int64_t* req_pos,
                           uint64_t nreq, void* out, int32_t* status, hipStream_t stream) {
   if (nreq == 0) return hipSuccess;
+  const PosDirEntry* dir = reinterpret_cast<const PosDirEntry*>(dir_words);
   const dim3 grid((unsigned)((nreq + 255) / 256)), block(256);
   // one workgroup's worth: the fallback kernel alone takes every request
   const bool one = nreq <= 256;
@@ -667,9 +730,10 @@ hipError_t utl_launch_get(bool u64, const uint8_t* src, const uint64_t* boff, co
   return hipGetLastError();
 }
 
-hipError_t utl_launch_pos_dir(const uint64_t* ooff, uint64_t nblocks, uint64_t* dir, uint64_t ndir,
-                              hipStream_t stream) {
+hipError_t utl_launch_pos_dir(const uint64_t* boff, const uint64_t* ooff, uint64_t nblocks, uint64_t* dir_words,
+                              uint64_t ndir, hipStream_t stream) {
   if (nblocks == 0 || ndir == 0) return hipSuccess;
-  k_pos_dir<<<dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, stream>>>(ooff, nblocks, dir, ndir);
+  k_pos_dir<<<dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, stream>>>(
+      boff, ooff, nblocks, reinterpret_cast<PosDirEntry*>(dir_words), ndir);
   return hipGetLastError();
 }
