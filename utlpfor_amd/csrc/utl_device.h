@@ -11,13 +11,6 @@
 
 namespace utl {
 
-struct __attribute__((packed, aligned(4))) Dw2 {
-  uint32_t x, y;
-};
-struct __attribute__((packed, aligned(4))) Dw3 {
-  uint32_t x, y, z;
-};
-
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -41,20 +34,6 @@ __device__ __forceinline__ uint64_t ld_u64(const uint8_t* p) {
   return (uint64_t)ld_u32(p) | ((uint64_t)ld_u32(p + 4) << 32);
 }
 
-// 8 bytes at p, where (p & 3) == s is the same for the whole wave.
-__device__ __forceinline__ void ld_pair(const uint8_t* p, uint32_t s, uint32_t& a, uint32_t& b) {
-  const uint8_t* q = p - s;
-  if (s == 0) {
-    Dw2 d = *(const Dw2*)q;
-    a = d.x;
-    b = d.y;
-  } else {
-    Dw3 d = *(const Dw3*)q;
-    a = __builtin_amdgcn_alignbyte(d.y, d.x, s);
-    b = __builtin_amdgcn_alignbyte(d.z, d.y, s);
-  }
-}
-
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int t = lane_id();
@@ -76,12 +55,6 @@ __device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint32_t wave_reduce_or(uint32_t x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, 64);
-  return x;
-}
-
 __device__ __forceinline__ uint32_t wave_reduce_min(uint32_t x) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, 64));
@@ -94,29 +67,6 @@ __device__ __forceinline__ uint32_t wave_reduce_max(uint32_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint64_t wave_reduce_min64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    uint64_t y = __shfl_xor(x, d, 64);
-    x = y < x ? y : x;
-  }
-  return x;
-}
-
-__device__ __forceinline__ uint64_t wave_reduce_max64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    uint64_t y = __shfl_xor(x, d, 64);
-    x = y > x ? y : x;
-  }
-  return x;
-}
-
-__device__ __forceinline__ uint64_t wave_reduce_or64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, 64);
-  return x;
-}
 
 __device__ __forceinline__ int bit_len32(uint32_t x) { return x ? 32 - __clz((int)x) : 0; }
 
