@@ -291,6 +291,47 @@ def test_fuzz_encode_bytes(ctx, orc, seed, is64, method):
     s.free()
 
 
+@pytest.mark.parametrize("is64", [False, True])
+@pytest.mark.parametrize("method", ["slot", "direct", "sized"])
+def test_encode_exception_heavy_blocks(ctx, orc, is64, method):
+    """Blocks whose exception sections are the longest the cost model picks:
+    small values with up to 100 huge ones (4-byte StreamVByte codes, about
+    400 section bytes), 1- and 2-byte codes, the list/bitmap switch, at every
+    byte alignment of the section (short streams of random lengths between
+    them); bytes equal the oracle's stream by stream."""
+    from utlpfor_amd import batch
+    if is64 and method == "direct":
+        pytest.skip("the direct form is uint32 only")
+    rng = np.random.default_rng(77 + is64)
+    parts, flags = [], []
+    for k in range(240):
+        if k % 3 == 2:  # a short stream that shifts the next block's alignment
+            parts.append(rng.integers(0, 1 << int(rng.integers(1, 20)), int(rng.integers(1, 9)), dtype=np.uint64))
+        else:
+            n = 128 if rng.random() < 0.8 else int(rng.integers(100, 129))
+            v = rng.integers(0, 16, n, dtype=np.uint64)
+            nexc = int(rng.choice([1, 15, 16, 17, 33, 64, 90, 100]))
+            idx = rng.choice(n, min(nexc, n), replace=False)
+            hi = int(rng.choice([12, 20, 28, 32]))
+            v[idx] = rng.integers(1 << (hi - 1), 1 << hi, idx.size, dtype=np.uint64)
+            parts.append(v)
+        flags.append(int(rng.choice([0, 2])))  # FOR allowed or NoFOR; no delta, so outliers stay outliers
+    if not is64:
+        parts = [p.astype(np.uint32) for p in parts]
+    lens = np.array([p.size for p in parts], np.uint64)
+    say(f"exception-heavy encode {'u64' if is64 else 'u32'} {method}: {lens.size} streams")
+    s = batch.encode(ctx, np.concatenate(parts), stream_lengths=lens, stream_flags=np.array(flags, np.uint8),
+                     method=method)
+    got = s.bytes_host()
+    off = 0
+    for p, f in zip(parts, flags):
+        want = orc.pack_stream(int(f), p)[0]
+        assert np.array_equal(got[off:off + want.size], want), (int(f), p.size)
+        off += want.size
+    assert off == s.nbytes
+    s.free()
+
+
 def test_encode_forms_with_invalid_jobs(ctx, orc):
     """A job table with counts outside 1..128 (ErrInvalidBuffer: meta word
     0xFFFFFFFF, nothing written) between valid jobs: the slot and sized
