@@ -206,6 +206,12 @@ struct Tail {
 
 __device__ __forceinline__ uint32_t svb_code(uint32_t h) { return (uint32_t)((bit_len32(h | 1u) - 1) >> 3); }
 
+// Data bytes of the high bits x >> bw of an exception x >= 2^bw (its
+// StreamVByte code + 1): (bit_len(x) - bw + 7) >> 3.
+__device__ __forceinline__ uint32_t exc_bytes(uint32_t x, int bw) {
+  return (uint32_t)(39 - bw - __clz((int)x)) >> 3;
+}
+
 __device__ __forceinline__ void tail_plan(EncGrpLds* L, int blk, int j, const uint32_t* x0, const uint32_t* x1,
                                           const bool* ok0, const bool* ok1, int bw, int exc, Tail& t) {
   const uint32_t m = low_mask(bw);
@@ -221,8 +227,6 @@ __device__ __forceinline__ void tail_plan(EncGrpLds* L, int blk, int j, const ui
 #pragma unroll
   for (int d = 0; d < 4; ++d) bm[d] = g8_or(bm[d] << (2 * j));
   const uint32_t pre1 = __popc(bm[0]), pre2 = pre1 + __popc(bm[1]), pre3 = pre2 + __popc(bm[2]);
-  // bytes of the high bits x >> bw of an exception x >= 2^bw: (bit_len(x) - bw + 7) >> 3
-  const int nb_k = 39 - bw;
   // Data bytes (code + 1 each) of this thread's exceptions in row r: byte r
   // of lA (rows 0-3) or byte r - 4 of lB, those of its first value in fA,
   // fB. A row holds at most 16 exceptions, so every per-row sum below stays
@@ -231,8 +235,8 @@ __device__ __forceinline__ void tail_plan(EncGrpLds* L, int blk, int j, const ui
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const bool e0 = ok0[r] && x0[r] > m, e1 = ok1[r] && x1[r] > m;
-    const uint32_t l0 = e0 ? (uint32_t)(nb_k - __clz((int)x0[r])) >> 3 : 0u;
-    const uint32_t l = l0 + (e1 ? (uint32_t)(nb_k - __clz((int)x1[r])) >> 3 : 0u);
+    const uint32_t l0 = e0 ? exc_bytes(x0[r], bw) : 0u;
+    const uint32_t l = l0 + (e1 ? exc_bytes(x1[r], bw) : 0u);
     if (r < 4) {
       lA |= l << (8 * r);
       fA |= l0 << (8 * r);
@@ -632,8 +636,8 @@ __device__ __forceinline__ void encode_u32_wave(EncGrpLds* L, uint64_t* wg, uint
     uint32_t eb = 0;
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      eb += (ok0[r] && x0[r] > m) ? (uint32_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
-      eb += (ok1[r] && x1[r] > m) ? (uint32_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+      eb += (ok0[r] && x0[r] > m) ? exc_bytes(x0[r], bw) : 0u;
+      eb += (ok1[r] && x1[r] > m) ? exc_bytes(x1[r], bw) : 0u;
     }
     len = poff + 16 * bw + (has_exc ? isz + svb_ctrl_count(exc) + (int)g8_add(eb) : 0);
     tile_scan();
@@ -792,8 +796,8 @@ __device__ __forceinline__ void size_u32_wave(uint64_t b, bool mine, bool ok, in
   uint32_t eb = 0;  // StreamVByte data bytes of this thread's exceptions
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
-    eb += (ok0[r] && x0[r] > m) ? (uint32_t)(((bit_len32((x0[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
-    eb += (ok1[r] && x1[r] > m) ? (uint32_t)(((bit_len32((x1[r] >> bw) | 1u) - 1) >> 3) + 1) : 0u;
+    eb += (ok0[r] && x0[r] > m) ? exc_bytes(x0[r], bw) : 0u;
+    eb += (ok1[r] && x1[r] > m) ? exc_bytes(x1[r], bw) : 0u;
   }
   eb = g8_add(eb);
   const bool has_exc = exc > 0;
