@@ -222,6 +222,36 @@ def test_walk_many_streams_long_in_every_tile(ctx, orc):
     assert counts[1500] == len(st0) + 5000
 
 
+def test_walk_dense_short_streams(ctx, orc):
+    """Short streams of many small blocks (zeros: 4-byte blocks; values below
+    16: 68-byte blocks), which the plan hands to the wavefront walk, clean,
+    damaged (a middle header byte, a truncation, a zero run) and with a
+    first block unlike the rest (a small block before large ones, a large
+    one before small ones), between ordinary short streams and a long
+    stream of small blocks: the index equals the host walk stream by
+    stream, and so does the decode through it."""
+    rng = np.random.default_rng(31)
+    zeros = orc.pack_stream(0, np.zeros(100_000, np.uint32))[0].tobytes()
+    small = orc.pack_stream(2, rng.integers(0, 16, 45_000).astype(np.uint32))[0].tobytes()
+    big_after_small = (orc.pack_stream(0, np.zeros(128, np.uint32))[0].tobytes()
+                       + orc.pack_stream(0, rng.integers(0, 1 << 30, 800).astype(np.uint32))[0].tobytes())
+    small_after_big = (orc.pack_stream(0, rng.integers(0, 1 << 30, 128).astype(np.uint32))[0].tobytes()
+                       + orc.pack_stream(0, np.zeros(60_000, np.uint32))[0].tobytes())
+    long_small = orc.pack_stream(2, rng.integers(0, 16, 300_000).astype(np.uint32))[0].tobytes()
+    assert len(zeros) < 32640 and len(small) < 32640 and len(long_small) > 32640
+    hit = bytearray(zeros)
+    st = _starts(orc, zeros)
+    hit[st[len(st) // 2]] = 200  # count byte 200: ErrInvalidBlockLength
+    cut = small[: len(small) - 37]
+    run = small[:6800] + bytes(4000) + small[6800:]
+    shorts, _ = make_streams(orc, rng, 30, 3000, False)
+    streams = [zeros, small, bytes(hit), cut, run, big_after_small, small_after_big, long_small] + shorts
+    order = rng.permutation(len(streams))
+    streams = [streams[k] for k in order]
+    say(f"walk dense: {len(streams)} streams, {sum(map(len, streams))} bytes")
+    run_walk(ctx, orc, streams, False)
+
+
 def test_walk_large_input_longer_chunks(ctx, orc):
     """A walk over more than 113 MB cuts long streams into chunks longer than
     16,320 bytes (the chunk length grows with the square root of the bytes

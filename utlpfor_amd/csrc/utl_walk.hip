@@ -8,10 +8,15 @@
 // same blocks and statuses.
 //
 // Short streams (< kLong bytes): one lane per stream walks it, reading the
-// headers from global memory (k_walk_serial).
+// headers from global memory (k_walk_serial); a short stream whose first
+// block is short enough that its walk would be a long chain of dependent
+// loads (more than kDenseChain blocks) is walked by a wavefront through LDS
+// windows instead (k_walk_dense).
 //
-// Long streams are cut into chunks (walk_chunk_len bytes, at least kChunk)
-// and walked chunk-parallel:
+// Long streams are cut into chunks and walked chunk-parallel. A stream's
+// chunk length is the call's (walk_chunk_len bytes), shortened for a
+// stream whose first block is short, so that a chunk holds about
+// kChainBlocks blocks, but never below kChunk bytes:
 //   k_walk_vote    per chunk, a guess of its entry (the first block start at
 //                  or after the chunk start): every candidate position of a
 //                  kWarm-byte warm-up region before the chunk is followed by
@@ -57,6 +62,9 @@ constexpr uint64_t kNone = ~0ull;
 constexpr uint16_t kDead = 0xFFFF;
 constexpr uint32_t kStLand = 0, kStEnd = 1, kStStop = 0x100;  // chunk exit kinds (kStStop | error)
 constexpr int kSmallVer = 64;  // long streams of at most this many chunks are verified by 64 threads
+constexpr uint64_t kChainBlocks = 280;  // blocks a chunk is sized for, from the stream's first block
+constexpr uint64_t kDenseChain = 320;   // short streams of more blocks than this (estimated) walk by wavefront
+constexpr uint32_t kDense = 1;          // clen of a short stream walked by wavefront (else 0)
 
 struct WalkRec {
   uint64_t e;     // entry (stream-relative), kNone if the vote found none
@@ -385,17 +393,36 @@ __device__ __forceinline__ int32_t status_of(uint32_t st) {
 constexpr int kPlan = 1024;
 
 // chunk_base[s] = chunks of the streams of s's tile before s (s <= ns; the
-// entry ns closes the last stream); tile_sum[t] = tile t's total.
-__global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict__ stream_off, uint64_t ns,
+// entry ns closes the last stream); tile_sum[t] = tile t's total; clen[s] =
+// the chunk length of a long stream, kDense for a short stream walked by
+// wavefront, 0 for the others. The lengths come from each stream's first
+// block (BlockLength where its header passes the checks that do not depend
+// on the bytes left; no estimate otherwise): only the chunking depends on
+// them, never the result.
+__global__ __launch_bounds__(kPlan) void k_walk_plan(const uint8_t* __restrict__ src,
+                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                      uint64_t chunk, uint64_t* __restrict__ chunk_base,
-                                                     uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ list,
+                                                     uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ clen,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ dlist,
                                                      uint32_t* __restrict__ cnt) {
   __shared__ uint64_t red[kPlan / 64];
   const int tid = threadIdx.x, wid = tid >> 6;
   const uint64_t s = (uint64_t)blockIdx.x * kPlan + (uint64_t)tid;
   uint64_t so;
   const uint64_t Ls = s < ns ? stream_len(stream_off, s, &so) : 0;
-  const uint64_t c = Ls >= kLong ? (Ls + chunk - 1) / chunk : 0;
+  uint64_t bl0 = 0;  // the first block's length, 0 when unknown
+  if (Ls >= 16) {
+    uint32_t h, d1, d2;
+    hdr96(load96(src, so, Ls, 0), h, d1, d2);
+    bool ok;
+    const uint32_t bl = fast_len(h, d1, d2, &ok);
+    bl0 = ok ? bl : 0;
+  }
+  uint64_t cl = chunk;
+  if (Ls >= kLong && bl0) cl = min(chunk, max((uint64_t)kChunk, (kChainBlocks * bl0) & ~15ull));
+  const uint64_t c = Ls >= kLong ? (Ls + cl - 1) / cl : 0;
+  const bool dense = Ls < kLong && bl0 && Ls > kDenseChain * bl0;
+  if (s < ns) clen[s] = c ? (uint32_t)cl : dense ? kDense : 0u;
   const uint64_t incl = wave_incl_scan64(c);
   if (lane_id() == 63) red[wid] = incl;
   __syncthreads();
@@ -408,21 +435,25 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint64_t* __restrict_
   if (s <= ns) chunk_base[s] = before + incl - c;
   if (tid == 0) tile_sum[blockIdx.x] = sum;
   // the long streams by verification class: list[0 ..) those of at most
-  // kSmallVer chunks, list[ns - 1] downwards the others (one atomic per
-  // wavefront and class)
+  // kSmallVer chunks, list[ns - 1] downwards the others; the dense short
+  // streams in dlist (one atomic per wavefront and class)
   const int ln = lane_id();
   const uint64_t lt = (1ull << ln) - 1;
   const uint64_t bs = __ballot(c != 0 && c <= (uint64_t)kSmallVer), bb = __ballot(c > (uint64_t)kSmallVer);
-  uint32_t a = 0, b = 0;
+  const uint64_t bd = __ballot(dense);
+  uint32_t a = 0, b = 0, e = 0;
   if (ln == 0 && bs) a = atomicAdd(&cnt[0], (uint32_t)__popcll(bs));
   if (ln == 0 && bb) b = atomicAdd(&cnt[1], (uint32_t)__popcll(bb));
+  if (ln == 0 && bd) e = atomicAdd(&cnt[2], (uint32_t)__popcll(bd));
   a = (uint32_t)__builtin_amdgcn_readfirstlane((int)a);
   b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+  e = (uint32_t)__builtin_amdgcn_readfirstlane((int)e);
   if ((bs >> ln) & 1) list[a + (uint32_t)__popcll(bs & lt)] = (uint32_t)s;
   if ((bb >> ln) & 1) list[ns - 1 - (b + (uint32_t)__popcll(bb & lt))] = (uint32_t)s;
+  if ((bd >> ln) & 1) dlist[e + (uint32_t)__popcll(bd & lt)] = (uint32_t)s;
 }
 
-__global__ void k_walk_zero(uint32_t* cnt) { cnt[threadIdx.x] = 0u; }
+__global__ void k_walk_zero(uint32_t* cnt) { cnt[threadIdx.x] = 0u; }  // the three list counters
 
 // Exclusive scan of q[0..n) in place (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
@@ -476,7 +507,7 @@ static_assert(kVotePer == 8 && kLand == kWarm, "a vote thread owns 8 consecutive
 __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ src,
                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                   uint64_t chunk, WalkRec* __restrict__ recs) {
+                                                   const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs) {
   __shared__ uint4 W[kWarm16];
   __shared__ uint4 J4[kWarm / 8];  // J[p]: next hop of position p (16-bit entries)
   __shared__ uint32_t hist[kLand];
@@ -497,7 +528,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
       if (tid == 0) recs[c].e = 0;
       continue;
     }
-    const uint64_t cs = k * chunk;  // chunk start (stream-relative)
+    const uint64_t cs = k * clen[s];  // chunk start (stream-relative)
     const uint64_t ws = cs - kWarm;            // warm-up start
     const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
     __syncthreads();
@@ -588,7 +619,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
 __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                    uint64_t chunk, WalkRec* __restrict__ recs) {
+                                                    const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs) {
   // one lane per chunk: the chunks' chains of dependent header loads run
   // side by side instead of one wavefront stepping through one chunk
   const uint64_t total = min(chunk_base[ns], max_chunks);
@@ -601,7 +632,7 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
     const uint64_t k = c - chunk_base[s];
     const uint64_t e = recs[c].e;
     WalkRes r{kNone, 0u, kStLand};
-    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * chunk, Ls), nullptr, nullptr, 0);
+    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * clen[s], Ls), nullptr, nullptr, 0);
     recs[c].x = r.x;
     recs[c].n = r.n;
     recs[c].st = r.st;
@@ -629,7 +660,8 @@ __device__ __forceinline__ uint32_t wg_min(uint32_t x, uint32_t* red) {
 template <int NT>
 __global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
-                                                    const uint64_t* __restrict__ chunk_base, uint64_t chunk,
+                                                    const uint64_t* __restrict__ chunk_base,
+                                                    const uint32_t* __restrict__ clen,
                                                     const uint32_t* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                     WalkRec* __restrict__ recs, uint32_t* __restrict__ term,
                                                     uint32_t* __restrict__ nblocks, int32_t* __restrict__ status) {
@@ -693,7 +725,7 @@ __global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ 
       E = fb == 0 ? E : xs[fb - 1];
       k += fb;
       if (wid == 0) {
-        const WalkRes w = walk_chunk(W, src, so, Ls, chunk, k, E, nullptr, nullptr, 0);
+        const WalkRes w = walk_chunk(W, src, so, Ls, clen[s], k, E, nullptr, nullptr, 0);
         if (lane_id() == 0) {
           WalkRec f{E, w.x, w.n, w.st, total, 0u};
           recs[c0 + k] = f;
@@ -726,7 +758,7 @@ __global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ 
 __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ src,
                                                   const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                   const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                  uint64_t chunk, const WalkRec* __restrict__ recs,
+                                                  const uint32_t* __restrict__ clen, const WalkRec* __restrict__ recs,
                                                   const uint32_t* __restrict__ term,
                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
                                                   uint64_t* __restrict__ start) {
@@ -741,14 +773,16 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
     if (k > term[s]) continue;
     const WalkRec r = recs[c];
     if (r.n == 0) continue;
-    walk_lane_out(src, so, Ls, r.e, min((k + 1) * chunk, Ls), meta, start, blk_off[s] + r.base);
+    walk_lane_out(src, so, Ls, r.e, min((k + 1) * clen[s], Ls), meta, start, blk_off[s] + r.base);
   }
 }
 
-// Serial walk of the short streams (one lane per stream).
+// Serial walk of the short streams (one lane per stream); with a plan
+// (chunk_base set), not the long streams nor the dense short ones.
 __global__ __launch_bounds__(256) void k_walk_serial(const uint8_t* __restrict__ src,
                                                      const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                      const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                     const uint32_t* __restrict__ clen,
                                                      const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
                                                      uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
                                                      int32_t* __restrict__ status) {
@@ -756,10 +790,32 @@ __global__ __launch_bounds__(256) void k_walk_serial(const uint8_t* __restrict__
   for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += step) {
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
-    if (chunk_base && is_long(Ls, chunk_base, s, max_chunks)) continue;
+    if (chunk_base && (is_long(Ls, chunk_base, s, max_chunks) || (Ls < kLong && clen[s] == kDense))) continue;
     const WalkRes r = walk_lane_out(src, so, Ls, 0, Ls, meta, start, meta ? blk_off[s] : 0);
     nblocks[s] = r.n;
     status[s] = status_of(r.st);
+  }
+}
+
+// Walk of the dense short streams of dlist (one wavefront per stream,
+// through LDS windows: no chain of dependent global loads).
+__global__ __launch_bounds__(64) void k_walk_dense(const uint8_t* __restrict__ src,
+                                                   const uint64_t* __restrict__ stream_off,
+                                                   const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ cnt,
+                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
+                                                   uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
+                                                   int32_t* __restrict__ status) {
+  __shared__ uint4 W[kWin16];
+  const uint32_t nd = cnt[2];
+  for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) {
+    const uint64_t s = dlist[i];
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    const WalkRes r = walk_span(W, src, so, Ls, 0, Ls, meta, meta ? start : nullptr, meta ? blk_off[s] : 0);
+    if (lane_id() == 0) {
+      nblocks[s] = r.n;
+      status[s] = status_of(r.st);
+    }
   }
 }
 
@@ -780,15 +836,17 @@ static uint64_t walk_chunk_len(uint64_t total_bytes) {
 }
 
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
-// records (at most one per kChunk bytes plus one per long stream), term and
-// the long-stream lists (one word per stream each), two list counters.
+// records (at most one per kChunk bytes plus one per long stream: a chunk
+// is never shorter than kChunk), term, the long-stream lists, the chunk
+// lengths and the dense-stream list (one word per stream each), three list
+// counters.
 static uint64_t walk_words(uint64_t n, uint64_t bytes) { return (n * bytes + 255) / 256 * 256; }
 
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   return walk_words(nstreams + 1, 8) + walk_words(ntiles, 8) + max_chunks * sizeof(WalkRec) +
-         2 * walk_words(nstreams, 4) + 256;
+         4 * walk_words(nstreams, 4) + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
@@ -798,7 +856,6 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   if (nstreams > 0xFFFFFFFFull) return hipErrorInvalidValue;  // stream numbers are 32-bit words
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t chunk = walk_chunk_len(total_bytes);
-  const uint64_t est_chunks = std::min(total_bytes / chunk + total_bytes / kLong + 1, max_chunks);  // for grids
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   uint8_t* t = static_cast<uint8_t*>(tmp);
   uint64_t* chunk_base = reinterpret_cast<uint64_t*>(t);
@@ -811,35 +868,50 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   t += walk_words(nstreams, 4);
   uint32_t* list = reinterpret_cast<uint32_t*>(t);
   t += walk_words(nstreams, 4);
+  uint32_t* clen = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(nstreams, 4);
+  uint32_t* dlist = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(nstreams, 4);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(t);
-  const bool chunked = total_bytes >= kLong;
-  if (chunked) {
-    k_walk_zero<<<dim3(1), dim3(2), 0, stream>>>(cnt);
-    k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(stream_off, nstreams, chunk, chunk_base,
-                                                                     tile_sum, list, cnt);
+  // the plan (chunk lengths, dense short streams) pays from a few KiB on
+  const bool planned = total_bytes >= 4096;
+  if (planned) {
+    k_walk_zero<<<dim3(1), dim3(3), 0, stream>>>(cnt);
+    k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(src, stream_off, nstreams, chunk, chunk_base,
+                                                                     tile_sum, clen, list, dlist, cnt);
     if (ntiles > 1) {
       k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(tile_sum, ntiles);
       k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
     }
-    const unsigned vgrid = (unsigned)std::min(est_chunks, (uint64_t)4096);
-    k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+  }
+  if (planned && total_bytes >= kLong) {
+    // grids from the bound on chunks (their count is on the device): every
+    // chunk has its own lane; workgroups past the count exit at once
+    const unsigned vgrid = (unsigned)std::min(max_chunks, (uint64_t)4096);
+    k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
                                                        recs);
-    const unsigned lgrid = (unsigned)std::min((est_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
-    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+    const unsigned lgrid = (unsigned)std::min((max_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
+    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
                                                         recs);
     // grids: every workgroup resident at once (a stream's verification is
     // a few rounds of loads and barriers)
     const uint64_t nlong_max = total_bytes / kLong + 1;
     k_walk_verify<kSmallVer><<<dim3((unsigned)std::min(nlong_max, (uint64_t)8192)), dim3(kSmallVer), 0, stream>>>(
-        src, stream_off, nstreams, chunk_base, chunk, list, cnt, recs, term, nblocks, status);
+        src, stream_off, nstreams, chunk_base, clen, list, cnt, recs, term, nblocks, status);
     k_walk_verify<1024><<<dim3((unsigned)std::min(nlong_max, (uint64_t)512)), dim3(1024), 0, stream>>>(
-        src, stream_off, nstreams, chunk_base, chunk, list, cnt, recs, term, nblocks, status);
+        src, stream_off, nstreams, chunk_base, clen, list, cnt, recs, term, nblocks, status);
     if (meta)
-      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, chunk,
+      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
                                                         recs, term, blk_off, meta, start);
   }
+  if (planned) {
+    // dense short streams: at most one per kDenseChain * 4 + 1 bytes (a block is at least 4 bytes)
+    const uint64_t ndense_max = std::min(nstreams, total_bytes / (kDenseChain * 4 + 1) + 1);
+    k_walk_dense<<<dim3((unsigned)std::min(ndense_max, (uint64_t)4096)), dim3(64), 0, stream>>>(
+        src, stream_off, dlist, cnt, blk_off, meta, start, nblocks, status);
+  }
   const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
-  k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunked ? chunk_base : nullptr,
-                                                         max_chunks, blk_off, meta, start, nblocks, status);
+  k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, planned ? chunk_base : nullptr,
+                                                         max_chunks, clen, blk_off, meta, start, nblocks, status);
   return hipGetLastError();
 }
