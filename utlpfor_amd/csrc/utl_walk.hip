@@ -296,13 +296,42 @@ __device__ __forceinline__ void hdr96(const Raw96& r, uint32_t& h, uint32_t& d1,
 // or the meta word and the block's absolute start.
 enum LaneOut { kCount = 0, kMeta = 1, kMetaStart = 2 };
 
+// A lane walk's last outputs, stored four blocks at a time: global loads
+// and stores share one in-order counter, so a store issued before the next
+// header's load delays it; with the stores batched, three steps in four
+// wait for their load alone. q0 is the newest entry.
+struct OutQ {
+  uint32_t m0, m1, m2, m3;
+  uint64_t s0, s1, s2, s3;
+  uint32_t k;  // entries held (0..3 between steps)
+};
+
+// Stores the k newest entries of q, the newest at index last.
+template <int OUT>
+__device__ __forceinline__ void outq_flush(const OutQ& q, uint32_t k, uint32_t* __restrict__ meta,
+                                           uint64_t* __restrict__ start, uint64_t last) {
+  if (OUT >= kMeta) {
+    if (k > 0) meta[last] = q.m0;
+    if (k > 1) meta[last - 1] = q.m1;
+    if (k > 2) meta[last - 2] = q.m2;
+    if (k > 3) meta[last - 3] = q.m3;
+  }
+  if (OUT >= kMetaStart) {
+    if (k > 0) start[last] = q.s0;
+    if (k > 1) start[last - 1] = q.s1;
+    if (k > 2) start[last - 2] = q.s2;
+    if (k > 3) start[last - 3] = q.s3;
+  }
+}
+
 // One block of a lane walk: the header in r (at pos) is checked, the next
-// header's load is issued, then the block's outputs are stored. Returns
-// whether the walk goes on (st is set when it stops).
+// header's load is issued, then the block's outputs are queued (and the
+// queue stored when it holds four). Returns whether the walk goes on (st
+// is set when it stops).
 template <int OUT>
 __device__ __forceinline__ bool lane_step(const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t end,
                                           uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out,
-                                          Raw96& r, uint64_t& pos, uint32_t& n, uint32_t& st) {
+                                          Raw96& r, uint64_t& pos, uint32_t& n, uint32_t& st, OutQ& q) {
   uint32_t h, d1, d2;
   hdr96(r, h, d1, d2);
   const uint64_t left = Ls - pos;
@@ -321,10 +350,23 @@ __device__ __forceinline__ bool lane_step(const uint8_t* __restrict__ src, uint6
   }
   const uint64_t next = pos + bl;
   const bool more = next < end && next != Ls;
-  const uint32_t m = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
   if (more) r = load96(src, so, Ls, next);
-  if (OUT >= kMeta) meta[out + n] = m;
-  if (OUT >= kMetaStart) start[out + n] = so + pos;
+  if (OUT >= kMeta) {
+    q.m3 = q.m2;
+    q.m2 = q.m1;
+    q.m1 = q.m0;
+    q.m0 = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
+    if (OUT >= kMetaStart) {
+      q.s3 = q.s2;
+      q.s2 = q.s1;
+      q.s1 = q.s0;
+      q.s0 = so + pos;
+    }
+    if (++q.k == 4) {
+      outq_flush<OUT>(q, 4, meta, start, out + n);
+      q.k = 0;
+    }
+  }
   ++n;
   pos = next;
   if (!more && pos == Ls) st = kStEnd;
@@ -344,10 +386,15 @@ __device__ WalkRes walk_lane(const uint8_t* __restrict__ src, uint64_t so, uint6
                              uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
   uint32_t n = 0, st = kStLand;
   if (pos >= end) return {pos, n, st};
+  OutQ q;
+  q.m0 = q.m1 = q.m2 = q.m3 = 0u;
+  q.s0 = q.s1 = q.s2 = q.s3 = 0u;
+  q.k = 0u;
   Raw96 r = load96(src, so, Ls, pos);
-  if (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st))
-    while (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st)) {
+  if (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st, q))
+    while (lane_step<OUT>(src, so, Ls, end, meta, start, out, r, pos, n, st, q)) {
     }
+  if (OUT >= kMeta && q.k) outq_flush<OUT>(q, q.k, meta, start, out + n - 1);
   return {pos, n, st};
 }
 
