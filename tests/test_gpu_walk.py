@@ -224,12 +224,14 @@ def test_walk_many_streams_long_in_every_tile(ctx, orc):
 
 def test_walk_dense_short_streams(ctx, orc):
     """Short streams of many small blocks (zeros: 4-byte blocks; values below
-    16: 68-byte blocks), which the plan hands to the wavefront walk, clean,
-    damaged (a middle header byte, a truncation, a zero run) and with a
-    first block unlike the rest (a small block before large ones, a large
-    one before small ones), between ordinary short streams and a long
-    stream of small blocks: the index equals the host walk stream by
-    stream, and so does the decode through it."""
+    16: 68-byte blocks), which the plan hands to the workgroup walk by
+    pointer jumping, clean, damaged (a middle header byte, a truncation, a
+    zero run) and with a first block unlike the rest (a small block before
+    large ones, a large one before small ones), between ordinary short
+    streams, a long stream of 68-byte blocks and long streams of 4-byte
+    blocks (chunks walked by workgroups), one of those damaged in a middle
+    header: the index equals the host walk stream by stream, and so does
+    the decode through it."""
     rng = np.random.default_rng(31)
     zeros = orc.pack_stream(0, np.zeros(100_000, np.uint32))[0].tobytes()
     small = orc.pack_stream(2, rng.integers(0, 16, 45_000).astype(np.uint32))[0].tobytes()
@@ -238,14 +240,18 @@ def test_walk_dense_short_streams(ctx, orc):
     small_after_big = (orc.pack_stream(0, rng.integers(0, 1 << 30, 128).astype(np.uint32))[0].tobytes()
                        + orc.pack_stream(0, np.zeros(60_000, np.uint32))[0].tobytes())
     long_small = orc.pack_stream(2, rng.integers(0, 16, 300_000).astype(np.uint32))[0].tobytes()
-    assert len(zeros) < 32640 and len(small) < 32640 and len(long_small) > 32640
+    long_zeros = orc.pack_stream(0, np.zeros(1_500_000, np.uint32))[0].tobytes()
+    assert len(zeros) < 32640 and len(small) < 32640 and len(long_small) > 32640 and len(long_zeros) > 32640
+    long_hit = bytearray(long_zeros)
+    long_hit[_starts(orc, long_zeros)[7000]] = 200  # ErrInvalidBlockLength in the second chunk
     hit = bytearray(zeros)
     st = _starts(orc, zeros)
     hit[st[len(st) // 2]] = 200  # count byte 200: ErrInvalidBlockLength
     cut = small[: len(small) - 37]
     run = small[:6800] + bytes(4000) + small[6800:]
     shorts, _ = make_streams(orc, rng, 30, 3000, False)
-    streams = [zeros, small, bytes(hit), cut, run, big_after_small, small_after_big, long_small] + shorts
+    streams = [zeros, small, bytes(hit), cut, run, big_after_small, small_after_big, long_small, long_zeros,
+               bytes(long_hit)] + shorts
     order = rng.permutation(len(streams))
     streams = [streams[k] for k in order]
     say(f"walk dense: {len(streams)} streams, {sum(map(len, streams))} bytes")

@@ -10,8 +10,8 @@
 // Short streams (< kLong bytes): one lane per stream walks it, reading the
 // headers from global memory (k_walk_serial); a short stream whose first
 // block is short enough that its walk would be a long chain of dependent
-// loads (more than kDenseChain blocks) is walked by a wavefront through LDS
-// windows instead (k_walk_dense).
+// loads (more than kDenseChain blocks) is walked by a workgroup, window by
+// window, by pointer jumping instead (k_walk_dense).
 //
 // Long streams are cut into chunks and walked chunk-parallel. A stream's
 // chunk length is the call's (walk_chunk_len bytes), shortened for a
@@ -64,7 +64,10 @@ constexpr uint32_t kStLand = 0, kStEnd = 1, kStStop = 0x100;  // chunk exit kind
 constexpr int kSmallVer = 64;  // long streams of at most this many chunks are verified by 64 threads
 constexpr uint64_t kChainBlocks = 280;  // blocks a chunk is sized for, from the stream's first block
 constexpr uint64_t kDenseChain = 320;   // short streams of more blocks than this (estimated) walk by wavefront
-constexpr uint32_t kDense = 1;          // clen of a short stream walked by wavefront (else 0)
+constexpr uint32_t kDense = 1;          // clen of a short stream walked by a workgroup (else 0)
+constexpr uint32_t kClenPJ = 0x80000000u;  // clen flag: the stream's chunks are walked by workgroups
+
+__device__ __forceinline__ uint64_t chunk_len_of(uint32_t cl) { return cl & ~kClenPJ; }
 
 struct WalkRec {
   uint64_t e;     // entry (stream-relative), kNone if the vote found none
@@ -469,7 +472,10 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint8_t* __restrict__
   if (Ls >= kLong && bl0) cl = min(chunk, max((uint64_t)kChunk, (kChainBlocks * bl0) & ~15ull));
   const uint64_t c = Ls >= kLong ? (Ls + cl - 1) / cl : 0;
   const bool dense = Ls < kLong && bl0 && Ls > kDenseChain * bl0;
-  if (s < ns) clen[s] = c ? (uint32_t)cl : dense ? kDense : 0u;
+  // blocks so small that even a kChunk-byte chunk is a long chain: the
+  // chunks are walked by workgroups (pointer jumping) instead of lanes
+  const bool pj = c && bl0 && kChainBlocks * bl0 < (uint64_t)kChunk;
+  if (s < ns) clen[s] = c ? (uint32_t)cl | (pj ? kClenPJ : 0u) : dense ? kDense : 0u;
   const uint64_t incl = wave_incl_scan64(c);
   if (lane_id() == 63) red[wid] = incl;
   __syncthreads();
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint8_t* __restrict__
   if ((bd >> ln) & 1) dlist[e + (uint32_t)__popcll(bd & lt)] = (uint32_t)s;
 }
 
-__global__ void k_walk_zero(uint32_t* cnt) { cnt[threadIdx.x] = 0u; }  // the three list counters
+__global__ void k_walk_zero(uint32_t* cnt) { cnt[threadIdx.x] = 0u; }  // the four list counters
 
 // Exclusive scan of q[0..n) in place (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_walk_scan(uint64_t* q, uint64_t n) {
@@ -575,7 +581,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
       if (tid == 0) recs[c].e = 0;
       continue;
     }
-    const uint64_t cs = k * clen[s];  // chunk start (stream-relative)
+    const uint64_t cs = k * chunk_len_of(clen[s]);  // chunk start (stream-relative)
     const uint64_t ws = cs - kWarm;            // warm-up start
     const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
     __syncthreads();
@@ -666,7 +672,8 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
 __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
-                                                    const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs) {
+                                                    const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs,
+                                                    uint32_t* __restrict__ pjlist, uint32_t* __restrict__ cnt) {
   // one lane per chunk: the chunks' chains of dependent header loads run
   // side by side instead of one wavefront stepping through one chunk
   const uint64_t total = min(chunk_base[ns], max_chunks);
@@ -676,10 +683,15 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint32_t cl = clen[s];
+    if (cl & kClenPJ) {  // walked by k_walk_chunks_pj, which takes the chunks listed here
+      pjlist[atomicAdd(&cnt[3], 1u)] = (uint32_t)c;
+      continue;
+    }
     const uint64_t k = c - chunk_base[s];
     const uint64_t e = recs[c].e;
     WalkRes r{kNone, 0u, kStLand};
-    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * clen[s], Ls), nullptr, nullptr, 0);
+    if (e != kNone) r = walk_lane<kCount>(src, so, Ls, e, min((k + 1) * chunk_len_of(cl), Ls), nullptr, nullptr, 0);
     recs[c].x = r.x;
     recs[c].n = r.n;
     recs[c].st = r.st;
@@ -772,7 +784,7 @@ __global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ 
       E = fb == 0 ? E : xs[fb - 1];
       k += fb;
       if (wid == 0) {
-        const WalkRes w = walk_chunk(W, src, so, Ls, clen[s], k, E, nullptr, nullptr, 0);
+        const WalkRes w = walk_chunk(W, src, so, Ls, chunk_len_of(clen[s]), k, E, nullptr, nullptr, 0);
         if (lane_id() == 0) {
           WalkRec f{E, w.x, w.n, w.st, total, 0u};
           recs[c0 + k] = f;
@@ -816,11 +828,13 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint32_t cl = clen[s];
+    if (cl & kClenPJ) continue;  // k_walk_chunks_pj
     const uint64_t k = c - chunk_base[s];
     if (k > term[s]) continue;
     const WalkRec r = recs[c];
     if (r.n == 0) continue;
-    walk_lane_out(src, so, Ls, r.e, min((k + 1) * clen[s], Ls), meta, start, blk_off[s] + r.base);
+    walk_lane_out(src, so, Ls, r.e, min((k + 1) * chunk_len_of(cl), Ls), meta, start, blk_off[s] + r.base);
   }
 }
 
@@ -844,24 +858,214 @@ __global__ __launch_bounds__(256) void k_walk_serial(const uint8_t* __restrict__
   }
 }
 
-// Walk of the dense short streams of dlist (one wavefront per stream,
-// through LDS windows: no chain of dependent global loads).
-__global__ __launch_bounds__(64) void k_walk_dense(const uint8_t* __restrict__ src,
-                                                   const uint64_t* __restrict__ stream_off,
-                                                   const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ cnt,
-                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
-                                                   uint64_t* __restrict__ start, uint32_t* __restrict__ nblocks,
-                                                   int32_t* __restrict__ status) {
-  __shared__ uint4 W[kWin16];
+// Window walk by pointer jumping (a workgroup of kPJThreads threads). Every
+// position p of a window of kPJ stream positions gets the position of the
+// block after a block starting at p (J0; kJOut when that lies at or past
+// the window's limit, kJEnd when the block ends the stream, kJDead when no
+// block can start at p: the reference's checks fail or the block is longer
+// than the bytes left). From the window's first position, a block start,
+// round k marks the chain positions at distances [2^k, 2^(k+1)) through the
+// table of 2^k jumps and builds the table of 2^(k+1) jumps (tables hold only
+// chain positions or terminals, and every jump moves forward, so a mark set
+// early in a round is still a chain position and the rounds end). When the
+// first position jumps to a terminal, every chain position is marked; the
+// marked positions other than a kJDead one are the window's blocks, in
+// order, and the last marked position tells how the walk goes on (the next
+// window, the walk's end, the stream end or a stop).
+constexpr int kPJ = 4096;                 // positions of a window
+constexpr int kPJ16 = (kPJ + 32) / 16;    // its staged bytes (12 header bytes past the last position)
+constexpr int kPJThreads = 256;
+constexpr int kPJPer = kPJ / kPJThreads;  // consecutive positions per thread
+constexpr uint16_t kJOut = kPJ, kJEnd = 0xFFFE, kJDead = 0xFFFF;
+static_assert(kPJ16 * 16 >= kPJ + 16 + 16, "a window holds 12 header bytes past its last position");
+
+struct PJLds {
+  uint4 W[kPJ16];
+  uint16_t J0[kPJ], JA[kPJ], JB[kPJ];
+  uint8_t mark[kPJ];
+  uint32_t red[kPJThreads / 64];
+  uint32_t last_p;
+};
+
+// BlockLength at window byte i (stream bytes left from it: left >= 1) as
+// the lane walk checks it; err is the stop status when no block can start
+// there (UTL_OK otherwise).
+__device__ __forceinline__ uint32_t pj_len(const uint32_t* W32, int i, uint64_t left, int& err, uint32_t& h) {
+  uint32_t d1, d2;
+  win96(W32, i, h, d1, d2);
+  bool ok;
+  uint32_t bl = fast_len(h, d1, d2, &ok);
+  err = UTL_OK;
+  if (!ok || left < 16u) err = step_len(h, d1, d2, (int64_t)min(left, (uint64_t)0x7FFFFFFF), &bl);
+  if (err == UTL_OK && bl > left) err = UTL_ERR_INVALID_BUFFER;
+  return bl;
+}
+
+// walk_lane's walk (stream at absolute so, length Ls) from the block start
+// pos while pos < end, by windows; with meta set, block i's meta word goes
+// to meta[out + i] and, with start set, its absolute start to start[out +
+// i]. Every thread of the workgroup calls it with the same arguments.
+__device__ WalkRes pj_walk(PJLds& L, const uint8_t* __restrict__ src, uint64_t so, uint64_t Ls, uint64_t pos,
+                           uint64_t end, uint32_t* __restrict__ meta, uint64_t* __restrict__ start, uint64_t out) {
+  const int tid = threadIdx.x, wid = tid >> 6;
+  const int p0 = tid * kPJPer;
+  uint32_t n = 0, st = kStLand;
+  while (pos < end) {
+    const int lim = (int)min(end - pos, (uint64_t)kPJ);  // positions of this window
+    const uint64_t wabs = stage<kPJ16, kPJThreads>(L.W, src, so + pos, so + min(pos + kPJ + 16, Ls), tid);
+    __syncthreads();
+    const uint32_t* W32 = reinterpret_cast<const uint32_t*>(L.W);
+    const int a0 = (int)(so + pos - wabs);
+#pragma unroll 4
+    for (int u = 0; u < kPJPer; ++u) {
+      const int p = p0 + u;
+      uint16_t j = kJDead;
+      if (p < lim) {  // (p < end - pos <= Ls - pos)
+        const uint64_t left = Ls - pos - (uint64_t)p;
+        int err;
+        uint32_t h;
+        const uint32_t bl = pj_len(W32, a0 + p, left, err, h);
+        if (err == UTL_OK) j = bl == left ? kJEnd : (uint32_t)p + bl >= (uint32_t)lim ? kJOut : (uint16_t)(p + bl);
+      }
+      L.J0[p] = j;
+      L.JA[p] = j;
+      L.mark[p] = p == 0 ? 1 : 0;
+    }
+    __syncthreads();
+    uint16_t* cur = L.JA;
+    uint16_t* nxt = L.JB;
+    for (;;) {
+#pragma unroll 4
+      for (int u = 0; u < kPJPer; ++u) {
+        const int p = p0 + u;
+        const uint16_t t = cur[p];
+        if (L.mark[p] && t < kPJ) L.mark[t] = 1;
+        nxt[p] = t < kPJ ? cur[t] : t;
+      }
+      __syncthreads();
+      const bool done = nxt[0] >= kPJ;
+      uint16_t* tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+      if (done) break;
+    }
+    // the window's blocks: marked positions other than a kJDead one; the
+    // last marked position has a terminal J0
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int u = 0; u < kPJPer; ++u) {
+      const int p = p0 + u;
+      if (L.mark[p]) {
+        const uint16_t j = L.J0[p];
+        c += j != kJDead ? 1u : 0u;
+        if (j >= kPJ) L.last_p = (uint32_t)p;
+      }
+    }
+    const uint32_t incl = wave_incl_scan(c);
+    if (lane_id() == 63) L.red[wid] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kPJThreads / 64; ++w) {
+      before += w < wid ? L.red[w] : 0u;
+      total += L.red[w];
+    }
+    if (meta) {
+      uint64_t k = out + n + before + incl - c;  // this thread's first block
+      for (int u = 0; u < kPJPer; ++u) {
+        const int p = p0 + u;
+        if (L.mark[p] && L.J0[p] != kJDead) {
+          int err;
+          uint32_t h;
+          const uint32_t bl = pj_len(W32, a0 + p, Ls - pos - (uint64_t)p, err, h);
+          meta[k] = (bl & 0xFFFFFFu) | ((h & 0xFFu) << 24);
+          if (start) start[k] = so + pos + (uint64_t)p;
+          ++k;
+        }
+      }
+    }
+    n += total;
+    const int lp = (int)L.last_p;
+    const uint16_t lj = L.J0[lp];
+    int err;
+    uint32_t h;
+    const uint32_t bl = pj_len(W32, a0 + lp, Ls - pos - (uint64_t)lp, err, h);
+    __syncthreads();  // W, the tables, the marks and red are rewritten by the next window
+    if (lj == kJDead) {  // (the walk stops at the position that is not a block, as walk_lane's)
+      pos += (uint64_t)lp;
+      st = kStStop | (uint32_t)err;
+      break;
+    }
+    if (lj == kJEnd) {
+      pos = Ls;
+      st = kStEnd;
+      break;
+    }
+    pos += (uint64_t)lp + bl;  // kJOut: the block crossing the window's limit
+  }
+  return {pos, n, st};
+}
+
+// Walk of the dense short streams of dlist, one workgroup per stream.
+__global__ __launch_bounds__(kPJThreads) void k_walk_dense(const uint8_t* __restrict__ src,
+                                                           const uint64_t* __restrict__ stream_off,
+                                                           const uint32_t* __restrict__ dlist,
+                                                           const uint32_t* __restrict__ cnt,
+                                                           const uint64_t* __restrict__ blk_off,
+                                                           uint32_t* __restrict__ meta, uint64_t* __restrict__ start,
+                                                           uint32_t* __restrict__ nblocks,
+                                                           int32_t* __restrict__ status) {
+  __shared__ PJLds L;
   const uint32_t nd = cnt[2];
   for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) {
     const uint64_t s = dlist[i];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
-    const WalkRes r = walk_span(W, src, so, Ls, 0, Ls, meta, meta ? start : nullptr, meta ? blk_off[s] : 0);
-    if (lane_id() == 0) {
+    const WalkRes r = pj_walk(L, src, so, Ls, 0, Ls, meta, meta ? start : nullptr, meta ? blk_off[s] : 0);
+    if (threadIdx.x == 0) {
       nblocks[s] = r.n;
       status[s] = status_of(r.st);
+    }
+  }
+}
+
+// The chunks of the streams flagged kClenPJ (listed by k_walk_chunks in
+// pjlist, cnt[3] of them), one workgroup per chunk: with meta unset,
+// k_walk_chunks' walk from the voted entry (block count, exit, stop status
+// into the chunk's record); with meta set, k_walk_emit's (the live chunks'
+// meta words and starts at their verified bases).
+__global__ __launch_bounds__(kPJThreads) void k_walk_chunks_pj(
+    const uint8_t* __restrict__ src, const uint64_t* __restrict__ stream_off, uint64_t ns,
+    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks, const uint32_t* __restrict__ clen,
+    const uint32_t* __restrict__ pjlist, const uint32_t* __restrict__ cnt, WalkRec* __restrict__ recs,
+    const uint32_t* __restrict__ term, const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
+    uint64_t* __restrict__ start) {
+  __shared__ PJLds L;
+  const uint32_t np = cnt[3];
+  for (uint32_t i = blockIdx.x; i < np; i += gridDim.x) {
+    const uint64_t c = pjlist[i];
+    const uint64_t s = owner(chunk_base, ns, c);
+    uint64_t so;
+    const uint64_t Ls = stream_len(stream_off, s, &so);
+    if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
+    const uint32_t cl = clen[s];
+    if (!(cl & kClenPJ)) continue;
+    const uint64_t k = c - chunk_base[s];
+    const uint64_t end = min((k + 1) * chunk_len_of(cl), Ls);
+    if (!meta) {
+      const uint64_t e = recs[c].e;
+      WalkRes r{kNone, 0u, kStLand};
+      if (e != kNone) r = pj_walk(L, src, so, Ls, e, end, nullptr, nullptr, 0);
+      if (threadIdx.x == 0) {
+        recs[c].x = r.x;
+        recs[c].n = r.n;
+        recs[c].st = r.st;
+      }
+    } else {
+      if (k > term[s]) continue;
+      const WalkRec r = recs[c];
+      if (r.n == 0) continue;
+      pj_walk(L, src, so, Ls, r.e, end, meta, start, blk_off[s] + r.base);
     }
   }
 }
@@ -885,15 +1089,15 @@ static uint64_t walk_chunk_len(uint64_t total_bytes) {
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
 // records (at most one per kChunk bytes plus one per long stream: a chunk
 // is never shorter than kChunk), term, the long-stream lists, the chunk
-// lengths and the dense-stream list (one word per stream each), three list
-// counters.
+// lengths and the dense-stream list (one word per stream each), the list of
+// chunks walked by workgroups (one word per chunk), four list counters.
 static uint64_t walk_words(uint64_t n, uint64_t bytes) { return (n * bytes + 255) / 256 * 256; }
 
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   return walk_words(nstreams + 1, 8) + walk_words(ntiles, 8) + max_chunks * sizeof(WalkRec) +
-         4 * walk_words(nstreams, 4) + 256;
+         4 * walk_words(nstreams, 4) + walk_words(max_chunks, 4) + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
@@ -919,11 +1123,13 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   t += walk_words(nstreams, 4);
   uint32_t* dlist = reinterpret_cast<uint32_t*>(t);
   t += walk_words(nstreams, 4);
+  uint32_t* pjlist = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(max_chunks, 4);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(t);
   // the plan (chunk lengths, dense short streams) pays from a few KiB on
   const bool planned = total_bytes >= 4096;
   if (planned) {
-    k_walk_zero<<<dim3(1), dim3(3), 0, stream>>>(cnt);
+    k_walk_zero<<<dim3(1), dim3(4), 0, stream>>>(cnt);
     k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(src, stream_off, nstreams, chunk, chunk_base,
                                                                      tile_sum, clen, list, dlist, cnt);
     if (ntiles > 1) {
@@ -939,7 +1145,13 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
                                                        recs);
     const unsigned lgrid = (unsigned)std::min((max_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
     k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
-                                                        recs);
+                                                        recs, pjlist, cnt);
+    // the chunks of small-block streams: a workgroup each (their count is on
+    // the device; 1024 workgroups are resident at once)
+    const unsigned pjgrid = (unsigned)std::min(max_chunks, (uint64_t)1024);
+    k_walk_chunks_pj<<<dim3(pjgrid), dim3(kPJThreads), 0, stream>>>(src, stream_off, nstreams, chunk_base,
+                                                                    max_chunks, clen, pjlist, cnt, recs, nullptr,
+                                                                    nullptr, nullptr, nullptr);
     // grids: every workgroup resident at once (a stream's verification is
     // a few rounds of loads and barriers)
     const uint64_t nlong_max = total_bytes / kLong + 1;
@@ -947,14 +1159,18 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
         src, stream_off, nstreams, chunk_base, clen, list, cnt, recs, term, nblocks, status);
     k_walk_verify<1024><<<dim3((unsigned)std::min(nlong_max, (uint64_t)512)), dim3(1024), 0, stream>>>(
         src, stream_off, nstreams, chunk_base, clen, list, cnt, recs, term, nblocks, status);
-    if (meta)
+    if (meta) {
       k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
                                                         recs, term, blk_off, meta, start);
+      k_walk_chunks_pj<<<dim3(pjgrid), dim3(kPJThreads), 0, stream>>>(src, stream_off, nstreams, chunk_base,
+                                                                      max_chunks, clen, pjlist, cnt, recs, term,
+                                                                      blk_off, meta, start);
+    }
   }
   if (planned) {
     // dense short streams: at most one per kDenseChain * 4 + 1 bytes (a block is at least 4 bytes)
     const uint64_t ndense_max = std::min(nstreams, total_bytes / (kDenseChain * 4 + 1) + 1);
-    k_walk_dense<<<dim3((unsigned)std::min(ndense_max, (uint64_t)4096)), dim3(64), 0, stream>>>(
+    k_walk_dense<<<dim3((unsigned)std::min(ndense_max, (uint64_t)1024)), dim3(kPJThreads), 0, stream>>>(
         src, stream_off, dlist, cnt, blk_off, meta, start, nblocks, status);
   }
   const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
