@@ -416,16 +416,6 @@ __device__ __forceinline__ uint64_t stream_len(const uint64_t* stream_off, uint6
   return b > a ? b - a : 0;
 }
 
-// The stream owning chunk c: the largest s < ns with chunk_base[s] <= c.
-__device__ uint64_t owner(const uint64_t* chunk_base, uint64_t ns, uint64_t c) {
-  uint64_t lo = 0, hi = ns - 1;
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi + 1) >> 1;
-    if (chunk_base[mid] <= c) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
 
 // A stream is walked chunk-parallel when it is long and its chunks fit the
 // record scratch (max_chunks); every kernel applies this same test.
@@ -539,6 +529,27 @@ __global__ __launch_bounds__(kPlan) void k_walk_apply(uint64_t* __restrict__ chu
   if (s <= ns) chunk_base[s] += tile_off[t];
 }
 
+// owner_of[c] = the stream of chunk c, for every chunk of the long streams
+// (chunk_base final). A thread per stream fills a stream of at most 64
+// chunks; the wavefront fills the longer ones of its 64 streams together.
+__global__ __launch_bounds__(kPlan) void k_walk_owner(const uint64_t* __restrict__ chunk_base, uint64_t ns,
+                                                      uint64_t max_chunks, uint32_t* __restrict__ owner_of) {
+  const uint64_t s = (uint64_t)blockIdx.x * kPlan + threadIdx.x;
+  uint64_t a = 0, b = 0;
+  if (s < ns) {
+    a = min(chunk_base[s], max_chunks);
+    b = min(chunk_base[s + 1], max_chunks);
+  }
+  const bool big = b - a > 64;
+  if (!big)
+    for (uint64_t c = a; c < b; ++c) owner_of[c] = (uint32_t)s;
+  for (uint64_t m = __ballot(big); m; m &= m - 1) {  // (all 64 lanes)
+    const int l = __builtin_ctzll(m);
+    const uint64_t al = __shfl(a, l, 64), bl = __shfl(b, l, 64), sl = __shfl(s, l, 64);
+    for (uint64_t c = al + (uint64_t)lane_id(); c < bl; c += 64) owner_of[c] = (uint32_t)sl;
+  }
+}
+
 // Whether header h passes the checks of BlockLength that do not depend on
 // the bytes left (count, bit width, flags). A header failing them ends
 // every walk that reaches it.
@@ -560,6 +571,7 @@ static_assert(kVotePer == 8 && kLand == kWarm, "a vote thread owns 8 consecutive
 __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ src,
                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                   const uint32_t* __restrict__ owner_of,
                                                    const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs) {
   __shared__ uint4 W[kWarm16];
   __shared__ uint4 J4[kWarm / 8];  // J[p]: next hop of position p (16-bit entries)
@@ -572,7 +584,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
   // cleared once; each chunk clears the entries it counted in
   for (int i = tid; i < kLand; i += 256) hist[i] = 0;
   for (uint64_t c = blockIdx.x; c < total; c += gridDim.x) {
-    const uint64_t s = owner(chunk_base, ns, c);
+    const uint64_t s = owner_of[c];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
@@ -672,6 +684,7 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
 __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ src,
                                                     const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                     const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                    const uint32_t* __restrict__ owner_of,
                                                     const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs,
                                                     uint32_t* __restrict__ pjlist, uint32_t* __restrict__ cnt) {
   // one lane per chunk: the chunks' chains of dependent header loads run
@@ -679,7 +692,7 @@ __global__ __launch_bounds__(64) void k_walk_chunks(const uint8_t* __restrict__ 
   const uint64_t total = min(chunk_base[ns], max_chunks);
   const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += step) {
-    const uint64_t s = owner(chunk_base, ns, c);
+    const uint64_t s = owner_of[c];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
@@ -817,6 +830,7 @@ __global__ __launch_bounds__(NT) void k_walk_verify(const uint8_t* __restrict__ 
 __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ src,
                                                   const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                   const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
+                                                  const uint32_t* __restrict__ owner_of,
                                                   const uint32_t* __restrict__ clen, const WalkRec* __restrict__ recs,
                                                   const uint32_t* __restrict__ term,
                                                   const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
@@ -824,7 +838,7 @@ __global__ __launch_bounds__(64) void k_walk_emit(const uint8_t* __restrict__ sr
   const uint64_t total = min(chunk_base[ns], max_chunks);
   const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += step) {
-    const uint64_t s = owner(chunk_base, ns, c);
+    const uint64_t s = owner_of[c];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
@@ -1036,15 +1050,16 @@ __global__ __launch_bounds__(kPJThreads) void k_walk_dense(const uint8_t* __rest
 // meta words and starts at their verified bases).
 __global__ __launch_bounds__(kPJThreads) void k_walk_chunks_pj(
     const uint8_t* __restrict__ src, const uint64_t* __restrict__ stream_off, uint64_t ns,
-    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks, const uint32_t* __restrict__ clen,
-    const uint32_t* __restrict__ pjlist, const uint32_t* __restrict__ cnt, WalkRec* __restrict__ recs,
+    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks, const uint32_t* __restrict__ owner_of,
+    const uint32_t* __restrict__ clen, const uint32_t* __restrict__ pjlist, const uint32_t* __restrict__ cnt,
+    WalkRec* __restrict__ recs,
     const uint32_t* __restrict__ term, const uint64_t* __restrict__ blk_off, uint32_t* __restrict__ meta,
     uint64_t* __restrict__ start) {
   __shared__ PJLds L;
   const uint32_t np = cnt[3];
   for (uint32_t i = blockIdx.x; i < np; i += gridDim.x) {
     const uint64_t c = pjlist[i];
-    const uint64_t s = owner(chunk_base, ns, c);
+    const uint64_t s = owner_of[c];
     uint64_t so;
     const uint64_t Ls = stream_len(stream_off, s, &so);
     if (!is_long(Ls, chunk_base, s, max_chunks)) continue;
@@ -1089,15 +1104,16 @@ static uint64_t walk_chunk_len(uint64_t total_bytes) {
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
 // records (at most one per kChunk bytes plus one per long stream: a chunk
 // is never shorter than kChunk), term, the long-stream lists, the chunk
-// lengths and the dense-stream list (one word per stream each), the list of
-// chunks walked by workgroups (one word per chunk), four list counters.
+// lengths and the dense-stream list (one word per stream each), the chunks'
+// streams and the list of chunks walked by workgroups (one word per chunk
+// each), four list counters.
 static uint64_t walk_words(uint64_t n, uint64_t bytes) { return (n * bytes + 255) / 256 * 256; }
 
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   return walk_words(nstreams + 1, 8) + walk_words(ntiles, 8) + max_chunks * sizeof(WalkRec) +
-         4 * walk_words(nstreams, 4) + walk_words(max_chunks, 4) + 256;
+         4 * walk_words(nstreams, 4) + 2 * walk_words(max_chunks, 4) + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
@@ -1125,6 +1141,8 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   t += walk_words(nstreams, 4);
   uint32_t* pjlist = reinterpret_cast<uint32_t*>(t);
   t += walk_words(max_chunks, 4);
+  uint32_t* owner_of = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(max_chunks, 4);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(t);
   // the plan (chunk lengths, dense short streams) pays from a few KiB on
   const bool planned = total_bytes >= 4096;
@@ -1138,20 +1156,21 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
     }
   }
   if (planned && total_bytes >= kLong) {
+    k_walk_owner<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, max_chunks, owner_of);
     // grids from the bound on chunks (their count is on the device): every
     // chunk has its own lane; workgroups past the count exit at once
     const unsigned vgrid = (unsigned)std::min(max_chunks, (uint64_t)4096);
-    k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
-                                                       recs);
+    k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, owner_of,
+                                                       clen, recs);
     const unsigned lgrid = (unsigned)std::min((max_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
-    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
-                                                        recs, pjlist, cnt);
+    k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, owner_of,
+                                                        clen, recs, pjlist, cnt);
     // the chunks of small-block streams: a workgroup each (their count is on
     // the device; 1024 workgroups are resident at once)
     const unsigned pjgrid = (unsigned)std::min(max_chunks, (uint64_t)1024);
     k_walk_chunks_pj<<<dim3(pjgrid), dim3(kPJThreads), 0, stream>>>(src, stream_off, nstreams, chunk_base,
-                                                                    max_chunks, clen, pjlist, cnt, recs, nullptr,
-                                                                    nullptr, nullptr, nullptr);
+                                                                    max_chunks, owner_of, clen, pjlist, cnt, recs,
+                                                                    nullptr, nullptr, nullptr, nullptr);
     // grids: every workgroup resident at once (a stream's verification is
     // a few rounds of loads and barriers)
     const uint64_t nlong_max = total_bytes / kLong + 1;
@@ -1160,11 +1179,11 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
     k_walk_verify<1024><<<dim3((unsigned)std::min(nlong_max, (uint64_t)512)), dim3(1024), 0, stream>>>(
         src, stream_off, nstreams, chunk_base, clen, list, cnt, recs, term, nblocks, status);
     if (meta) {
-      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
-                                                        recs, term, blk_off, meta, start);
+      k_walk_emit<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks,
+                                                        owner_of, clen, recs, term, blk_off, meta, start);
       k_walk_chunks_pj<<<dim3(pjgrid), dim3(kPJThreads), 0, stream>>>(src, stream_off, nstreams, chunk_base,
-                                                                      max_chunks, clen, pjlist, cnt, recs, term,
-                                                                      blk_off, meta, start);
+                                                                      max_chunks, owner_of, clen, pjlist, cnt, recs,
+                                                                      term, blk_off, meta, start);
     }
   }
   if (planned) {
