@@ -16,7 +16,8 @@
 // Long streams are cut into chunks and walked chunk-parallel. A stream's
 // chunk length is the call's (walk_chunk_len bytes), shortened for a
 // stream whose first block is short, so that a chunk holds about
-// kChainBlocks blocks, but never below kChunk bytes:
+// kChainBlocks blocks, but never below kChunk bytes (k_walk_plan; then
+// k_walk_owner maps each chunk to its stream):
 //   k_walk_vote    per chunk, a guess of its entry (the first block start at
 //                  or after the chunk start): every candidate position of a
 //                  kWarm-byte warm-up region before the chunk is followed by
@@ -26,7 +27,9 @@
 //                  merge into the stream's own chain);
 //   k_walk_chunks  one lane per chunk walks it from the guess, reading the
 //                  headers from global memory: block count, exit (first
-//                  position at or past the chunk end) and stop status;
+//                  position at or past the chunk end) and stop status; the
+//                  chunks of streams of tiny blocks go to k_walk_chunks_pj
+//                  (a workgroup per chunk, pointer jumping);
 //   k_walk_verify  one workgroup per long stream (64 threads for streams of
 //                  at most 64 chunks, else 1024) checks, a workgroup's
 //                  width of chunks at a time, that each chunk's guess
@@ -36,7 +39,8 @@
 //                  chunk counts only once its entry is the true one, and the
 //                  first chunk's entry is the stream start.
 //   k_walk_emit    (second pass) one lane per live chunk walks it again and
-//                  writes its blocks' meta words and starts.
+//                  writes its blocks' meta words and starts (k_walk_chunks_pj
+//                  for the chunks of streams of tiny blocks).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
