@@ -438,8 +438,9 @@ constexpr int kPlan = 1024;
 
 // chunk_base[s] = chunks of the streams of s's tile before s (s <= ns; the
 // entry ns closes the last stream); tile_sum[t] = tile t's total; clen[s] =
-// the chunk length of a long stream, kDense for a short stream walked by
-// wavefront, 0 for the others. The lengths come from each stream's first
+// the chunk length of a long stream, kDense for a short stream walked by a
+// workgroup, 0 for the others; flen[s] = a long stream's first block length
+// (0 when unknown). The lengths come from each stream's first
 // block (BlockLength where its header passes the checks that do not depend
 // on the bytes left; no estimate otherwise): only the chunking depends on
 // them, never the result.
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint8_t* __restrict__
                                                      const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                      uint64_t chunk, uint64_t* __restrict__ chunk_base,
                                                      uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ clen,
-                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ dlist,
-                                                     uint32_t* __restrict__ cnt) {
+                                                     uint32_t* __restrict__ flen, uint32_t* __restrict__ list,
+                                                     uint32_t* __restrict__ dlist, uint32_t* __restrict__ cnt) {
   __shared__ uint64_t red[kPlan / 64];
   const int tid = threadIdx.x, wid = tid >> 6;
   const uint64_t s = (uint64_t)blockIdx.x * kPlan + (uint64_t)tid;
@@ -469,7 +470,10 @@ __global__ __launch_bounds__(kPlan) void k_walk_plan(const uint8_t* __restrict__
   // blocks so small that even a kChunk-byte chunk is a long chain: the
   // chunks are walked by workgroups (pointer jumping) instead of lanes
   const bool pj = c && bl0 && kChainBlocks * bl0 < (uint64_t)kChunk;
-  if (s < ns) clen[s] = c ? (uint32_t)cl | (pj ? kClenPJ : 0u) : dense ? kDense : 0u;
+  if (s < ns) {
+    clen[s] = c ? (uint32_t)cl | (pj ? kClenPJ : 0u) : dense ? kDense : 0u;
+    flen[s] = c ? (uint32_t)bl0 : 0u;
+  }
   const uint64_t incl = wave_incl_scan64(c);
   if (lane_id() == 63) red[wid] = incl;
   __syncthreads();
@@ -576,7 +580,9 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
                                                    const uint64_t* __restrict__ stream_off, uint64_t ns,
                                                    const uint64_t* __restrict__ chunk_base, uint64_t max_chunks,
                                                    const uint32_t* __restrict__ owner_of,
-                                                   const uint32_t* __restrict__ clen, WalkRec* __restrict__ recs) {
+                                                   const uint32_t* __restrict__ clen, const uint32_t* __restrict__ flen,
+                                                   WalkRec* __restrict__ recs) {
+  __shared__ uint64_t fast_e;
   __shared__ uint4 W[kWarm16];
   __shared__ uint4 J4[kWarm / 8];  // J[p]: next hop of position p (16-bit entries)
   __shared__ uint32_t hist[kLand];
@@ -598,6 +604,35 @@ __global__ __launch_bounds__(256) void k_walk_vote(const uint8_t* __restrict__ s
       continue;
     }
     const uint64_t cs = k * chunk_len_of(clen[s]);  // chunk start (stream-relative)
+    // A stream of blocks of one length L0 (its first block's) has its entry
+    // at the first multiple of L0 at or after cs: taken when the headers
+    // there and one block before both parse with length L0, without the
+    // vote (a guess like the vote's; the verification decides).
+    const uint32_t L0 = flen[s];
+    if (L0 && L0 <= (uint32_t)kLand) {
+      if (tid == 0) {
+        const uint64_t g = (cs + L0 - 1) / L0 * L0;
+        bool hit = g + 16 <= Ls;
+        if (hit) {
+          const Raw96 ra = load96(src, so, Ls, g - L0), rb = load96(src, so, Ls, g);
+          uint32_t h, d1, d2;
+          bool oka, okb;
+          hdr96(ra, h, d1, d2);
+          const uint32_t la = fast_len(h, d1, d2, &oka);
+          hdr96(rb, h, d1, d2);
+          const uint32_t lb = fast_len(h, d1, d2, &okb);
+          hit = oka && okb && la == L0 && lb == L0;
+        }
+        fast_e = hit ? g : kNone;
+      }
+      __syncthreads();
+      const uint64_t fe = fast_e;
+      __syncthreads();  // (fast_e is rewritten for the next chunk)
+      if (fe != kNone) {
+        if (tid == 0) recs[c].e = fe;
+        continue;
+      }
+    }
     const uint64_t ws = cs - kWarm;            // warm-up start
     const uint64_t wabs = stage<kWarm16, 256>(W, src, so + ws, so + min(cs + 32, Ls), tid);
     __syncthreads();
@@ -1108,16 +1143,16 @@ static uint64_t walk_chunk_len(uint64_t total_bytes) {
 // Scratch layout: chunk_base (nstreams + 1 entries), tile sums, chunk
 // records (at most one per kChunk bytes plus one per long stream: a chunk
 // is never shorter than kChunk), term, the long-stream lists, the chunk
-// lengths and the dense-stream list (one word per stream each), the chunks'
-// streams and the list of chunks walked by workgroups (one word per chunk
-// each), four list counters.
+// lengths, the first block lengths and the dense-stream list (one word per
+// stream each), the chunks' streams and the list of chunks walked by
+// workgroups (one word per chunk each), four list counters.
 static uint64_t walk_words(uint64_t n, uint64_t bytes) { return (n * bytes + 255) / 256 * 256; }
 
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
   const uint64_t ntiles = (nstreams + 1 + kPlan - 1) / kPlan;
   return walk_words(nstreams + 1, 8) + walk_words(ntiles, 8) + max_chunks * sizeof(WalkRec) +
-         4 * walk_words(nstreams, 4) + 2 * walk_words(max_chunks, 4) + 256;
+         5 * walk_words(nstreams, 4) + 2 * walk_words(max_chunks, 4) + 256;
 }
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
@@ -1141,6 +1176,8 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   t += walk_words(nstreams, 4);
   uint32_t* clen = reinterpret_cast<uint32_t*>(t);
   t += walk_words(nstreams, 4);
+  uint32_t* flen = reinterpret_cast<uint32_t*>(t);
+  t += walk_words(nstreams, 4);
   uint32_t* dlist = reinterpret_cast<uint32_t*>(t);
   t += walk_words(nstreams, 4);
   uint32_t* pjlist = reinterpret_cast<uint32_t*>(t);
@@ -1153,7 +1190,7 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
   if (planned) {
     k_walk_zero<<<dim3(1), dim3(4), 0, stream>>>(cnt);
     k_walk_plan<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(src, stream_off, nstreams, chunk, chunk_base,
-                                                                     tile_sum, clen, list, dlist, cnt);
+                                                                     tile_sum, clen, flen, list, dlist, cnt);
     if (ntiles > 1) {
       k_walk_scan<<<dim3(1), dim3(1024), 0, stream>>>(tile_sum, ntiles);
       k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
@@ -1165,7 +1202,7 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
     // chunk has its own lane; workgroups past the count exit at once
     const unsigned vgrid = (unsigned)std::min(max_chunks, (uint64_t)4096);
     k_walk_vote<<<dim3(vgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, owner_of,
-                                                       clen, recs);
+                                                       clen, flen, recs);
     const unsigned lgrid = (unsigned)std::min((max_chunks + 63) / 64, (uint64_t)16384);  // one lane per chunk
     k_walk_chunks<<<dim3(lgrid), dim3(64), 0, stream>>>(src, stream_off, nstreams, chunk_base, max_chunks, owner_of,
                                                         clen, recs, pjlist, cnt);
