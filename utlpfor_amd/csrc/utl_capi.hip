@@ -51,6 +51,8 @@ int hip_fail(hipError_t e, const char* what) {
 struct utl_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // a second stream for work that runs beside the context's (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint8_t* h_stage = nullptr;  // pinned, host-mapped, fine-grained (coherent)
   uint8_t* d_stage = nullptr;  // the device's address of h_stage
   void* d_tmp = nullptr;
@@ -80,6 +82,9 @@ int ctx_init(utl_ctx* c, int device) {
   c->cu_count = prop.multiProcessorCount;
   snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   // Per-call staging: kernels read a call's inputs from and write its
   // results to this host buffer directly (fine-grained, so no GPU cache
   // holds a stale line between calls): one launch and one synchronization
@@ -99,6 +104,10 @@ void ctx_free(utl_ctx* c) {
   if (c->d_tmp) hipFree(c->d_tmp);
   if (c->d_counter) hipFree(c->d_counter);
   if (c->h_stage) hipHostFree(c->h_stage);
+  if (c->side) hipStreamSynchronize(c->side);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -600,7 +609,7 @@ int utl_index_walk(utl_ctx* ctx, const uint8_t* d_src, const uint64_t* d_stream_
   const uint64_t total = ends[1] > ends[0] ? ends[1] - ends[0] : 0;
   if ((err = ensure_tmp(c, utl_walk_tmp_bytes(nstreams, total)))) return err;
   HIP_TRY(utl_launch_index_walk(d_src, d_stream_off, nstreams, total, d_blk_off, d_meta, d_block_start, d_nblocks,
-                                d_status, c->d_tmp, c->stream));
+                                d_status, c->d_tmp, c->stream, c->side, c->ev_fork, c->ev_join));
   return UTL_OK;
 }
 

@@ -51,11 +51,14 @@ hipError_t utl_launch_block_lengths(const uint8_t* src, const uint64_t* start, c
 
 // Walk of foreign streams (block starts unknown): counts, then meta words
 // (utl_walk.hip). total_bytes = stream_off[nstreams] - stream_off[0] sizes
-// the scratch tmp (utl_walk_tmp_bytes bytes).
+// the scratch tmp (utl_walk_tmp_bytes bytes). The short streams are walked
+// on `side` (forked from and joined back into `stream` with the two
+// events), alongside the chunk-parallel walk of the long ones.
 size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes);
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
                                  uint64_t total_bytes, const uint64_t* blk_off, uint32_t* meta, uint64_t* start,
-                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream);
+                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream,
+                                 hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join);
 
 hipError_t utl_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t stream);
 hipError_t utl_launch_fill_bytes(uint8_t* p, uint8_t v, uint64_t n, hipStream_t stream);

@@ -1157,7 +1157,8 @@ size_t utl_walk_tmp_bytes(uint64_t nstreams, uint64_t total_bytes) {
 
 hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off, uint64_t nstreams,
                                  uint64_t total_bytes, const uint64_t* blk_off, uint32_t* meta, uint64_t* start,
-                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream) {
+                                 uint32_t* nblocks, int32_t* status, void* tmp, hipStream_t stream,
+                                 hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join) {
   if (nstreams == 0) return hipSuccess;
   if (nstreams > 0xFFFFFFFFull) return hipErrorInvalidValue;  // stream numbers are 32-bit words
   const uint64_t max_chunks = total_bytes / kChunk + total_bytes / kLong + 1;
@@ -1196,6 +1197,33 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
       k_walk_apply<<<dim3((unsigned)(ntiles - 1)), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, tile_sum);
     }
   }
+  // The short streams (lane walks and dense streams) need only the plan:
+  // with many streams of a mean length that suggests long ones as well,
+  // they run on the side stream beside the long streams' vote, chunk walks,
+  // verification and emit, and the context's stream waits for them at the
+  // end (the fork and join cost about 20 us, more than a walk of a few
+  // streams gains).
+  const bool fork = planned && nstreams >= 1024 && total_bytes / nstreams >= kLong;
+  const hipStream_t sh = fork ? side : stream;  // the short streams' stream
+  if (fork) {
+    const hipError_t e = hipEventRecord(ev_fork, stream);
+    if (e != hipSuccess) return e;
+    const hipError_t e2 = hipStreamWaitEvent(side, ev_fork, 0);
+    if (e2 != hipSuccess) return e2;
+  }
+  if (planned) {
+    // dense short streams: at most one per kDenseChain * 4 + 1 bytes (a block is at least 4 bytes)
+    const uint64_t ndense_max = std::min(nstreams, total_bytes / (kDenseChain * 4 + 1) + 1);
+    k_walk_dense<<<dim3((unsigned)std::min(ndense_max, (uint64_t)1024)), dim3(kPJThreads), 0, sh>>>(
+        src, stream_off, dlist, cnt, blk_off, meta, start, nblocks, status);
+    const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
+    k_walk_serial<<<dim3(lsgrid), dim3(256), 0, sh>>>(src, stream_off, nstreams, chunk_base, max_chunks, clen,
+                                                      blk_off, meta, start, nblocks, status);
+  }
+  if (fork) {
+    const hipError_t e = hipEventRecord(ev_join, side);
+    if (e != hipSuccess) return e;
+  }
   if (planned && total_bytes >= kLong) {
     k_walk_owner<<<dim3((unsigned)ntiles), dim3(kPlan), 0, stream>>>(chunk_base, nstreams, max_chunks, owner_of);
     // grids from the bound on chunks (their count is on the device): every
@@ -1227,14 +1255,13 @@ hipError_t utl_launch_index_walk(const uint8_t* src, const uint64_t* stream_off,
                                                                       term, blk_off, meta, start);
     }
   }
-  if (planned) {
-    // dense short streams: at most one per kDenseChain * 4 + 1 bytes (a block is at least 4 bytes)
-    const uint64_t ndense_max = std::min(nstreams, total_bytes / (kDenseChain * 4 + 1) + 1);
-    k_walk_dense<<<dim3((unsigned)std::min(ndense_max, (uint64_t)1024)), dim3(kPJThreads), 0, stream>>>(
-        src, stream_off, dlist, cnt, blk_off, meta, start, nblocks, status);
+  if (fork) {
+    const hipError_t e = hipStreamWaitEvent(stream, ev_join, 0);
+    if (e != hipSuccess) return e;
+  } else if (!planned) {
+    const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
+    k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, nullptr, max_chunks, clen,
+                                                          blk_off, meta, start, nblocks, status);
   }
-  const unsigned lsgrid = (unsigned)std::min((nstreams + 255) / 256, (uint64_t)4096);  // one lane per stream
-  k_walk_serial<<<dim3(lsgrid), dim3(256), 0, stream>>>(src, stream_off, nstreams, planned ? chunk_base : nullptr,
-                                                         max_chunks, clen, blk_off, meta, start, nblocks, status);
   return hipGetLastError();
 }
