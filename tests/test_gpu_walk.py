@@ -299,6 +299,59 @@ def test_walk_large_input_longer_chunks(ctx, orc):
     s.free()
 
 
+def test_walk_many_streams_short_ones_beside(ctx, orc):
+    """A call of at least 1024 streams whose mean length is at least 32,640
+    bytes walks its short streams on the context's second stream beside the
+    long streams' chunk pipeline: 1500 streams (long ones of 12,000-40,000
+    values, short ones, dense short ones of zeros), mixed flags, encoded on
+    the device, with a middle block of a few long and short streams
+    damaged; the counts and statuses equal the oracle's walk and the other
+    streams' blocks equal the encoder's index."""
+    from utlpfor_amd import batch
+    from utlpfor_amd.device import DeviceArray
+    rng = np.random.default_rng(33)
+    ns = 1500
+    kind = rng.integers(0, 10, ns)
+    lens = np.where(kind < 6, rng.integers(12_000, 40_000, ns),
+                    np.where(kind < 9, rng.integers(1, 3000, ns), rng.integers(40_000, 200_000, ns))).astype(np.int64)
+    parts = []
+    for i in range(ns):
+        if kind[i] == 9:  # dense short streams: zeros, 4-byte blocks
+            parts.append(np.zeros(int(lens[i]), np.uint32))
+        else:
+            parts.append(rng.integers(0, 1 << int(rng.integers(4, 33)), int(lens[i]), dtype=np.uint64).astype(np.uint32))
+    flags = rng.choice([0, 1, 2, 4, 3], ns).astype(np.uint8)
+    s = batch.encode(ctx, np.concatenate(parts), stream_lengths=lens, stream_flags=flags)
+    data = s.bytes_host().copy()
+    boff, meta = s.block_off.to_numpy(), s.meta.to_numpy()
+    sb = batch.plan_blocks(lens).astype(np.int64)
+    stream_off = boff[sb]
+    sizes = np.diff(stream_off)
+    assert ns >= 1024 and stream_off[-1] // ns >= 32640, (stream_off[-1] // ns)
+    big = np.nonzero(sizes > 70_000)[0]
+    small = np.nonzero((sizes < 20_000) & (np.diff(sb) > 4))[0]
+    damaged = [int(i) for i in big[:4]] + [int(i) for i in small[:4]]
+    for i in damaged:
+        data[int(boff[sb[i] + (sb[i + 1] - sb[i]) // 2])] = 200
+    say(f"walk beside: {ns} streams, {s.nbytes} bytes, {s.nblocks} blocks, damaged {damaged}")
+    d = DeviceArray.from_numpy(np.concatenate([data, np.zeros(16, np.uint8)]), ctx)
+    w, counts, status = batch.index_foreign(ctx, d, stream_off)
+    want_n = np.diff(sb)
+    want_st = np.zeros(ns, np.int32)
+    for i in damaged:
+        st0, _, _, st = host_walk(orc, bytes(data[int(stream_off[i]):int(stream_off[i + 1])]))
+        want_n[i], want_st[i] = len(st0), st
+    assert np.array_equal(counts, want_n) and np.array_equal(status, want_st)
+    keep = np.concatenate([np.arange(sb[i], sb[i] + want_n[i]) for i in range(ns)])
+    nb = keep.size
+    assert w.nblocks == nb
+    assert np.array_equal(w.block_off.to_numpy(nb), boff[keep])
+    assert np.array_equal(w.meta.to_numpy(nb), meta[keep])
+    w.free()
+    d.free()
+    s.free()
+
+
 def test_walk_long_streams_u64(ctx, orc):
     rng = np.random.default_rng(25)
     ts = (int(1.7e18) + np.cumsum(rng.geometric(1 / 1000, 200000))).astype(np.uint64)
